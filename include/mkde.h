@@ -1,0 +1,189 @@
+/*
+ * mkde.h -- C ABI of the MI355X-native tiled Gaussian KDE library (libmkde.so).
+ *
+ * The reference (Guiming/mGPUKDE_tiled) has no library API: its GPU path is a
+ * set of host functions inside kde_cuda_kdtr_tiled.cu ("tiled" below) that
+ * main() calls in sequence, passing SamplePoints / AsciiRaster structs by value
+ * and per-GPU device pointers.  Each entry point here replaces one of those
+ * internal seams; the comment above it names the reference function (file:line)
+ * whose role it takes.  A host caller of the reference's GPU path binds these.
+ *
+ * Conventions
+ *   - Every function returns MKDE_OK (0) or a positive mkde_status code; the
+ *     message of the last failure on the calling thread is mkde_last_error().
+ *     The library never exits the process (the reference calls exit() on every
+ *     CUDA error, e.g. tiled:1725-1729).
+ *   - The caller owns every host buffer; the library owns device buffers, held
+ *     in a context (one HIP device, one HIP stream).  Host output pointers may
+ *     be NULL when the caller wants the result to stay resident on the device.
+ *   - Points keep the caller's order in every input and output array.
+ *   - Calls on one context must not overlap (one thread at a time per context);
+ *     separate contexts are independent.
+ *   - Floating point follows the reference CPU path: single-precision
+ *     coordinates, the cut-off predicate fl64(dx^2+dy^2) < 9*fl32(h*h) with
+ *     dx = fl32(x1-x0) evaluated exactly, FP32 compensated sums for densities
+ *     and edge-correction integrals, FP64 for log-likelihood sums.
+ */
+#ifndef MKDE_H
+#define MKDE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MKDE_OK = 0,
+    MKDE_E_ARG = 1,     /* invalid argument (null pointer, bad size, bad option) */
+    MKDE_E_HIP = 2,     /* HIP runtime failure */
+    MKDE_E_NOMEM = 3,   /* device or host allocation failed */
+    MKDE_E_STATE = 4,   /* a required input (points, mask, bandwidths) is not set */
+    MKDE_E_IO = 5,      /* file read/write/format error */
+    MKDE_E_RANGE = 6    /* a size exceeds a documented maximum */
+} mkde_status;
+
+/* Raster geometry of a mask, density tile or output raster.  Mirrors the
+ * geometry fields of the reference's AsciiRaster (AsciiRaster.h:9-33):
+ * row 0 is the north row, cell centres follow COL_TO_XCOORD/ROW_TO_YCOORD
+ * (Utilities.h:13-14) evaluated in double and stored to float. */
+typedef struct {
+    int64_t ncols;
+    int64_t nrows;
+    float xll;       /* xLLCorner */
+    float yll;       /* yLLCorner */
+    float cellsize;
+    float nodata;    /* noDataValue: a cell is valid iff value != nodata */
+} mkde_grid;
+
+typedef struct mkde_ctx mkde_ctx;
+
+const char *mkde_last_error(void);
+const char *mkde_version(void);
+
+/* Device context: one HIP device, one HIP stream (default priority).
+ * Replaces the reference's per-GPU stream set-up (tiled:367-400). */
+int mkde_ctx_create(int device, mkde_ctx **out);
+int mkde_ctx_destroy(mkde_ctx *ctx);
+int mkde_synchronize(mkde_ctx *ctx);
+
+/* SamplePoints upload (AllocateDeviceSamplePoints + CopyToDeviceSamplePoints,
+ * tiled:1713-1805).  Builds the device-resident cell-binned point index that
+ * replaces the reference's k-d tree (BuildGPUKDtree tiled:4260;
+ * CUDA_KDTree::CreateKDTree CUDA_KDtree.cu:26-102).  w == NULL means unit
+ * weights (SAMPLEWEIGHTS = 0, tiled:1694-1696).  Resets bandwidths, boundary
+ * distances and edge-correction weights (weights to 1, tiled:889-893). */
+int mkde_points_upload(mkde_ctx *ctx, int64_t n, const float *x, const float *y, const float *w);
+int mkde_num_points(mkde_ctx *ctx, int64_t *n);
+
+/* Edge-correction mask upload (the Mask AsciiRaster of tiled:412/429, copied to
+ * the device by CopyToDeviceAsciiRaster tiled:2316-2373). */
+int mkde_mask_upload(mkde_ctx *ctx, const mkde_grid *grid, const float *values);
+
+/* Rule-of-thumb bandwidth h0 = sqrtf(BandWidth2(points)) (tiled:64-68, 481;
+ * MeanCenter tiled:1563, StandardDistance2 tiled:1581), with the reference's
+ * float/double sums in point order. */
+int mkde_rule_of_thumb(mkde_ctx *ctx, float *h0);
+
+/* MarkBoundary (tiled:3956-4069; dMarkBoundary kde_kernel_kdtr.cu:1209-1281):
+ * flags[r*ncols+c] = 1 for boundary cells of the edge-correction mask, else 0.
+ * flags may be NULL.  *n_boundary (may be NULL) receives the count. */
+int mkde_mark_boundary(mkde_ctx *ctx, uint8_t *flags, int64_t *n_boundary);
+
+/* CalcDist2Boundary (tiled:4072-4137, CPU semantics 4091-4131): squared
+ * distance (double) from every point to the nearest boundary cell centre,
+ * DBL_MAX if there is no boundary cell.  Runs mkde_mark_boundary if needed. */
+int mkde_dist_to_boundary(mkde_ctx *ctx, double *dist);
+int mkde_set_dist_to_boundary(mkde_ctx *ctx, const double *dist);
+
+/* Bandwidths: hs[] of the reference (tiled:1-, AllocateBandwidths 2878,
+ * CopyToDeviceBandwidths; Hoption -1 reads them from the CSV, tiled:413-420). */
+int mkde_set_bandwidth_fixed(mkde_ctx *ctx, float h);
+int mkde_set_bandwidths(mkde_ctx *ctx, const float *h);
+int mkde_get_bandwidths(mkde_ctx *ctx, float *h);
+
+/* Edge-correction weights with the current bandwidths
+ * (EdgeCorrectionWeightsExact tiled:2694-2807; CalcEdgeCorrectionWeights
+ * kde_kernel_kdtr.cu:117-265).  Requires the mask; computes boundary distances
+ * first if they are not set.  ew may be NULL; *n_eval (may be NULL) receives
+ * the number of (point, valid cell) pairs inside the cut-off. */
+int mkde_ec_weights(mkde_ctx *ctx, float *ew, int64_t *n_eval);
+/* Set weights explicitly (1.0 everywhere when ew == NULL: EC off). */
+int mkde_set_ec_weights(mkde_ctx *ctx, const float *ew);
+int mkde_get_ec_weights(mkde_ctx *ctx, float *ew);
+
+/* Range-search candidate sets (KDtree::SearchRange KDtree.cu:471-537 as used by
+ * LogLikelihood tiled:3309-3334; DensityAtPointsKdtr kde_kernel_kdtr.cu:681-804):
+ * for each query point i, all j with fl64(dx^2+dy^2) < 9*fl32(h_i*h_i) using
+ * the current bandwidths.  CSR output: offsets[n+1] and idx[capacity], indexes
+ * ascending per query.  With idx == NULL only *total is computed. */
+int mkde_range_candidates(mkde_ctx *ctx, int64_t *offsets, int32_t *idx, int64_t capacity, int64_t *total);
+
+/* Density at the sample points with the current bandwidths and EC weights,
+ * each SOURCE point using its own bandwidth (the sample-point estimator of
+ * DensityAtPointsKdtr + dCopyDensityValues, kde_kernel_kdtr.cu:681-981, and
+ * the CPU scatter of LogLikelihood tiled:3312-3347, 3615-3647).
+ * loo = 0: den0 (self term included); loo = 1: den1 (self term excluded). */
+int mkde_density_at_points(mkde_ctx *ctx, int loo, float *den);
+
+/* Log-likelihood of K fixed bandwidths (LogLikelihood fixed-h,
+ * tiled:3214-3368), all candidates in one batched pass over point pairs.
+ * ec_on != 0 recomputes EC weights per candidate (UPDATEWEIGHTS).  logL[k] is
+ * the FP64 sum of log(max(den1, 1e-14)).  K <= 1024.  Leaves the context's
+ * bandwidths and EC weights unchanged. */
+int mkde_loo_loglik(mkde_ctx *ctx, int K, const float *h_cand, int ec_on, double *logL);
+
+/* Log-likelihood of the adaptive estimator for (h, alpha) pairs
+ * (LogLikelihood adaptive-h, tiled:3374-3671): EC(h) -> pilot density at h ->
+ * geometric mean -> h_i = h*(den0_i/g)^alpha -> EC(h_i) -> LOO with per-source
+ * bandwidths.  Leaves the context's bandwidths and EC weights unchanged. */
+int mkde_loo_loglik_adaptive(mkde_ctx *ctx, int K, const float *h, const float *alpha, int ec_on, double *logL);
+
+/* Abramson adaptive bandwidths (Hoption 2 final step, tiled:1081-1127 and
+ * CPU tiled:566-578): EC(h0) if ec_on, pilot density den0 at h0,
+ * g = exp(mean log den0), h_i = h0 * powf(den0_i / g, alpha).  Sets the
+ * context's bandwidths to h_i (and EC weights to those at h0).  h_out, den0_out
+ * may be NULL; *g_out may be NULL. */
+int mkde_abramson(mkde_ctx *ctx, float h0, float alpha, int ec_on, float *h_out, float *den0_out, double *g_out);
+
+/* MLE_FixedBandWidth (tiled:3044-3093): the reference's bracketing search on
+ * [h/20, 2h] with the LOO log-likelihood.  Likelihoods are compared in FP64.
+ * *n_iter (may be NULL) receives the iteration count. */
+int mkde_mle_fixed_bandwidth(mkde_ctx *ctx, float h, int ec_on, float *hopt, int *n_iter);
+
+/* hooke_jeeves (tiled:3726-3790) over (h, alpha) with the adaptive
+ * log-likelihood.  out3 = (h, alpha, Lmax). */
+int mkde_hooke_jeeves(mkde_ctx *ctx, float h0, float alpha0, float step_h, float step_a, int ec_on, float *out3,
+                      int *n_iter);
+
+/* Density raster of one tile (the tile loop of the GPU path, tiled:1233-1372
+ * mode 1 / 1373-1517 mode 0; KernelDesityEstimation_pPoints
+ * kde_kernel_kdtr.cu:461-574; CPU semantics tiled:707-733): for every valid
+ * cell of the tile (mask value != nodata; mask == NULL: all cells valid) the
+ * sum over points of K(h_i, d) * w_i * ew_i with the current bandwidths and EC
+ * weights; nodata cells keep the mask value.  Cell centres use the tile's own
+ * origin (N2).  out (host) receives nrows*ncols floats; may be NULL (the
+ * result stays on the device).  *n_eval (may be NULL) is not computed here. */
+int mkde_density_tile(mkde_ctx *ctx, const mkde_grid *tile, const float *mask, float *out);
+
+/* Exact number of (point, valid cell) pairs inside the cut-off for a tile
+ * (the algorithmic work unit of the density kernel), computed by a separate
+ * counting pass. */
+int mkde_count_density_pairs(mkde_ctx *ctx, const mkde_grid *tile, const float *mask, int64_t *n_pairs);
+
+/* Device-resident variants used for throughput measurement: the tile mask is
+ * uploaded once (mkde_tile_upload), mkde_density_tile_resident computes into a
+ * device buffer owned by the context, mkde_tile_download copies it out. */
+int mkde_tile_upload(mkde_ctx *ctx, const mkde_grid *tile, const float *mask);
+int mkde_density_tile_resident(mkde_ctx *ctx);
+int mkde_tile_download(mkde_ctx *ctx, float *out);
+
+/* Kernel timing of the last density launch set on the context's stream
+ * (milliseconds, HIP events around the dominant kernel). */
+int mkde_last_kernel_ms(mkde_ctx *ctx, const char *which, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MKDE_H */

@@ -1,0 +1,324 @@
+"""mgpukde_tiled_amd -- MI355X-native tiled Gaussian KDE (host mirror over libmkde).
+
+Python view of the C ABI in include/mkde.h.  Method names follow the reference
+functions of kde_cuda_kdtr_tiled.cu whose role each call takes (MarkBoundary,
+CalcDist2Boundary, EdgeCorrectionWeightsExact, LogLikelihood,
+MLE_FixedBandWidth, hooke_jeeves, ...), so parity tests read like tests of the
+reference's own functions.  Every compute call runs the HIP kernels of
+libmkde.so on the GPU; there is no CPU fallback: if the library or a GPU is
+missing, the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmkde.so")
+HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "mkde.h")
+
+MKDE_OK = 0
+STATUS_NAMES = {1: "MKDE_E_ARG", 2: "MKDE_E_HIP", 3: "MKDE_E_NOMEM", 4: "MKDE_E_STATE", 5: "MKDE_E_IO",
+                6: "MKDE_E_RANGE"}
+
+
+class MkdeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (STATUS_NAMES.get(code, str(code)), msg))
+        self.code = code
+
+
+class Grid(ctypes.Structure):
+    """mkde_grid: geometry of a mask / tile (AsciiRaster.h:9-33 geometry)."""
+
+    _fields_ = [
+        ("ncols", ctypes.c_int64),
+        ("nrows", ctypes.c_int64),
+        ("xll", ctypes.c_float),
+        ("yll", ctypes.c_float),
+        ("cellsize", ctypes.c_float),
+        ("nodata", ctypes.c_float),
+    ]
+
+    def __repr__(self):
+        return "Grid(ncols=%d, nrows=%d, xll=%r, yll=%r, cellsize=%r, nodata=%r)" % (
+            self.ncols, self.nrows, self.xll, self.yll, self.cellsize, self.nodata)
+
+
+def grid(ncols, nrows, xll, yll, cellsize, nodata=-9999.0):
+    return Grid(int(ncols), int(nrows), float(xll), float(yll), float(cellsize), float(nodata))
+
+
+_FP = ctypes.POINTER(ctypes.c_float)
+_DP = ctypes.POINTER(ctypes.c_double)
+_I64P = ctypes.POINTER(ctypes.c_int64)
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_U8P = ctypes.POINTER(ctypes.c_uint8)
+_IP = ctypes.POINTER(ctypes.c_int)
+_GP = ctypes.POINTER(Grid)
+_CTX = ctypes.c_void_p
+
+# name -> argtypes (restype is int status unless listed in _RESTYPES)
+SIGNATURES = {
+    "mkde_last_error": [],
+    "mkde_version": [],
+    "mkde_ctx_create": [ctypes.c_int, ctypes.POINTER(_CTX)],
+    "mkde_ctx_destroy": [_CTX],
+    "mkde_synchronize": [_CTX],
+    "mkde_points_upload": [_CTX, ctypes.c_int64, _FP, _FP, _FP],
+    "mkde_num_points": [_CTX, _I64P],
+    "mkde_mask_upload": [_CTX, _GP, _FP],
+    "mkde_rule_of_thumb": [_CTX, _FP],
+    "mkde_mark_boundary": [_CTX, _U8P, _I64P],
+    "mkde_dist_to_boundary": [_CTX, _DP],
+    "mkde_set_dist_to_boundary": [_CTX, _DP],
+    "mkde_set_bandwidth_fixed": [_CTX, ctypes.c_float],
+    "mkde_set_bandwidths": [_CTX, _FP],
+    "mkde_get_bandwidths": [_CTX, _FP],
+    "mkde_ec_weights": [_CTX, _FP, _I64P],
+    "mkde_set_ec_weights": [_CTX, _FP],
+    "mkde_get_ec_weights": [_CTX, _FP],
+    "mkde_range_candidates": [_CTX, _I64P, _I32P, ctypes.c_int64, _I64P],
+    "mkde_density_at_points": [_CTX, ctypes.c_int, _FP],
+    "mkde_loo_loglik": [_CTX, ctypes.c_int, _FP, ctypes.c_int, _DP],
+    "mkde_loo_loglik_adaptive": [_CTX, ctypes.c_int, _FP, _FP, ctypes.c_int, _DP],
+    "mkde_abramson": [_CTX, ctypes.c_float, ctypes.c_float, ctypes.c_int, _FP, _FP, _DP],
+    "mkde_mle_fixed_bandwidth": [_CTX, ctypes.c_float, ctypes.c_int, _FP, _IP],
+    "mkde_hooke_jeeves": [_CTX, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int, _FP,
+                          _IP],
+    "mkde_density_tile": [_CTX, _GP, _FP, _FP],
+    "mkde_count_density_pairs": [_CTX, _GP, _FP, _I64P],
+    "mkde_tile_upload": [_CTX, _GP, _FP],
+    "mkde_density_tile_resident": [_CTX],
+    "mkde_tile_download": [_CTX, _FP],
+    "mkde_last_kernel_ms": [_CTX, ctypes.c_char_p, _DP],
+}
+_RESTYPES = {"mkde_last_error": ctypes.c_char_p, "mkde_version": ctypes.c_char_p}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libmkde.so (built in-tree by mgpukde_tiled_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise ImportError("libmkde.so is not built (%s); run `python -m mgpukde_tiled_amd.build`" % path)
+        L = ctypes.CDLL(path)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != MKDE_OK:
+        raise MkdeError(status, _lib.mkde_last_error().decode(errors="replace"))
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Context:
+    """A device context (one HIP device, one stream) with resident state."""
+
+    def __init__(self, device=0):
+        load_library()
+        h = _CTX()
+        _check(_lib.mkde_ctx_create(device, ctypes.byref(h)))
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.mkde_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def synchronize(self):
+        _check(_lib.mkde_synchronize(self._h))
+
+    # -- inputs ---------------------------------------------------------
+    def points_upload(self, x, y, w=None):
+        """CopyToDeviceSamplePoints + index build (replaces BuildGPUKDtree)."""
+        x, y = _f32(x), _f32(y)
+        w = None if w is None else _f32(w)
+        self._x, self._y, self._w = x, y, w
+        _check(_lib.mkde_points_upload(self._h, len(x), _p(x, _FP), _p(y, _FP), _p(w, _FP)))
+        self.n = len(x)
+
+    def mask_upload(self, g, values):
+        v = _f32(values).reshape(-1)
+        if v.size != g.ncols * g.nrows:
+            raise ValueError("mask has %d values, grid needs %d" % (v.size, g.ncols * g.nrows))
+        _check(_lib.mkde_mask_upload(self._h, ctypes.byref(g), _p(v, _FP)))
+        self.mask_grid = g
+
+    # -- reference functions ---------------------------------------------
+    def BandWidth_rule_of_thumb(self):
+        """h0 = sqrtf(BandWidth2(points)) (tiled:64-68, 481)."""
+        h = ctypes.c_float()
+        _check(_lib.mkde_rule_of_thumb(self._h, ctypes.byref(h)))
+        return h.value
+
+    def MarkBoundary(self):
+        g = self.mask_grid
+        flags = np.zeros(g.ncols * g.nrows, np.uint8)
+        nb = ctypes.c_int64()
+        _check(_lib.mkde_mark_boundary(self._h, _p(flags, _U8P), ctypes.byref(nb)))
+        return flags.reshape(g.nrows, g.ncols), nb.value
+
+    def CalcDist2Boundary(self):
+        d = np.empty(self.n, np.float64)
+        _check(_lib.mkde_dist_to_boundary(self._h, _p(d, _DP)))
+        return d
+
+    def set_dist_to_boundary(self, d):
+        d = np.ascontiguousarray(d, np.float64)
+        _check(_lib.mkde_set_dist_to_boundary(self._h, _p(d, _DP)))
+
+    def set_bandwidth(self, h):
+        _check(_lib.mkde_set_bandwidth_fixed(self._h, float(h)))
+
+    def set_bandwidths(self, hs):
+        hs = _f32(hs)
+        _check(_lib.mkde_set_bandwidths(self._h, _p(hs, _FP)))
+
+    def get_bandwidths(self):
+        out = np.empty(self.n, np.float32)
+        _check(_lib.mkde_get_bandwidths(self._h, _p(out, _FP)))
+        return out
+
+    def EdgeCorrectionWeightsExact(self):
+        """EC weights with the current bandwidths; returns (ew, n_eval)."""
+        ew = np.empty(self.n, np.float32)
+        ne = ctypes.c_int64()
+        _check(_lib.mkde_ec_weights(self._h, _p(ew, _FP), ctypes.byref(ne)))
+        return ew, ne.value
+
+    def set_ec_weights(self, ew=None):
+        ew = None if ew is None else _f32(ew)
+        _check(_lib.mkde_set_ec_weights(self._h, _p(ew, _FP)))
+
+    def get_ec_weights(self):
+        out = np.empty(self.n, np.float32)
+        _check(_lib.mkde_get_ec_weights(self._h, _p(out, _FP)))
+        return out
+
+    def SearchRange_all(self):
+        """Candidate sets of every point (CSR: offsets, indexes ascending)."""
+        tot = ctypes.c_int64()
+        offs = np.zeros(self.n + 1, np.int64)
+        _check(_lib.mkde_range_candidates(self._h, _p(offs, _I64P), None, 0, ctypes.byref(tot)))
+        idx = np.zeros(max(tot.value, 1), np.int32)
+        _check(_lib.mkde_range_candidates(self._h, _p(offs, _I64P), _p(idx, _I32P), tot.value, ctypes.byref(tot)))
+        return offs, idx[: tot.value]
+
+    def DensityAtPoints(self, loo=False):
+        out = np.empty(self.n, np.float32)
+        _check(_lib.mkde_density_at_points(self._h, 1 if loo else 0, _p(out, _FP)))
+        return out
+
+    def LogLikelihood(self, h_candidates, edge_correction=True):
+        """FP64 LOO log-likelihood of each fixed bandwidth (one batched pass)."""
+        hc = _f32(np.atleast_1d(h_candidates))
+        out = np.empty(len(hc), np.float64)
+        _check(_lib.mkde_loo_loglik(self._h, len(hc), _p(hc, _FP), 1 if edge_correction else 0, _p(out, _DP)))
+        return out
+
+    def LogLikelihoodAdaptive(self, h, alpha, edge_correction=True):
+        h = _f32(np.atleast_1d(h))
+        a = _f32(np.broadcast_to(np.atleast_1d(alpha), h.shape))
+        out = np.empty(len(h), np.float64)
+        _check(_lib.mkde_loo_loglik_adaptive(self._h, len(h), _p(h, _FP), _p(a, _FP), 1 if edge_correction else 0,
+                                             _p(out, _DP)))
+        return out
+
+    def Abramson(self, h0, alpha=-0.5, edge_correction=True):
+        """Pilot at h0, g, h_i = h0*(den0_i/g)^alpha.  Returns (h, den0, g)."""
+        hs = np.empty(self.n, np.float32)
+        d0 = np.empty(self.n, np.float32)
+        g = ctypes.c_double()
+        _check(_lib.mkde_abramson(self._h, float(h0), float(alpha), 1 if edge_correction else 0, _p(hs, _FP),
+                                  _p(d0, _FP), ctypes.byref(g)))
+        return hs, d0, g.value
+
+    def MLE_FixedBandWidth(self, h, edge_correction=True):
+        r = ctypes.c_float()
+        it = ctypes.c_int()
+        _check(_lib.mkde_mle_fixed_bandwidth(self._h, float(h), 1 if edge_correction else 0, ctypes.byref(r),
+                                             ctypes.byref(it)))
+        return r.value, it.value
+
+    def hooke_jeeves(self, h0, alpha0=-0.5, step_h=None, step_a=0.1, edge_correction=True):
+        if step_h is None:
+            step_h = np.float32(h0) / np.float32(10)
+        out = np.zeros(3, np.float32)
+        it = ctypes.c_int()
+        _check(_lib.mkde_hooke_jeeves(self._h, float(h0), float(alpha0), float(step_h), float(step_a),
+                                      1 if edge_correction else 0, _p(out, _FP), ctypes.byref(it)))
+        return (float(out[0]), float(out[1]), float(out[2])), it.value
+
+    def KernelDensityEstimation_tile(self, tile, mask=None):
+        """Density raster of one tile (nodata cells keep the mask value)."""
+        m = None if mask is None else _f32(mask).reshape(-1)
+        out = np.empty(tile.ncols * tile.nrows, np.float32)
+        _check(_lib.mkde_density_tile(self._h, ctypes.byref(tile), _p(m, _FP), _p(out, _FP)))
+        return out.reshape(tile.nrows, tile.ncols)
+
+    def count_density_pairs(self, tile, mask=None):
+        m = None if mask is None else _f32(mask).reshape(-1)
+        n = ctypes.c_int64()
+        _check(_lib.mkde_count_density_pairs(self._h, ctypes.byref(tile), _p(m, _FP), ctypes.byref(n)))
+        return n.value
+
+    # -- device-resident tile API (throughput measurement) ----------------
+    def tile_upload(self, tile, mask=None):
+        m = None if mask is None else _f32(mask).reshape(-1)
+        self._tile = tile
+        _check(_lib.mkde_tile_upload(self._h, ctypes.byref(tile), _p(m, _FP)))
+
+    def density_tile_resident(self):
+        _check(_lib.mkde_density_tile_resident(self._h))
+
+    def tile_download(self):
+        t = self._tile
+        out = np.empty(t.ncols * t.nrows, np.float32)
+        _check(_lib.mkde_tile_download(self._h, _p(out, _FP)))
+        return out.reshape(t.nrows, t.ncols)
+
+    def last_kernel_ms(self, which="density"):
+        v = ctypes.c_double()
+        _check(_lib.mkde_last_kernel_ms(self._h, which.encode(), ctypes.byref(v)))
+        return v.value
+
+
+def declared_symbols(header=HEADER_PATH):
+    """Names of the functions declared in include/mkde.h."""
+    import re
+
+    txt = open(header).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int)\s*(mkde_\w+)\s*\(", txt, re.M)))
+
+
+def version():
+    load_library()
+    return _lib.mkde_version().decode()

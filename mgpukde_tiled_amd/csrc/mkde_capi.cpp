@@ -1,0 +1,1052 @@
+// mkde_capi.cpp -- host side of libmkde: device context, the cell-binned point
+// index, the job drivers behind the C ABI of include/mkde.h.
+//
+// The reference drives its GPU path from main() through host functions that
+// exit() on every error and stage every intermediate through the host
+// (Reform* round trips, tiled:4297-4646).  Here all state stays resident on
+// one device in a context; host copies happen only where the caller asks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "mkde.h"
+#include "mkde_device.h"
+
+namespace mkde_dev {
+hipError_t prep_points(hipStream_t, int64_t, const int *, const float *, const float *, const float *, const float *,
+                       const float *, float, PtRec *);
+hipError_t bin_stats(hipStream_t, int, const int *, const PtRec *, BinInfo *, double *);
+hipError_t density_tile(hipStream_t, const GridDesc &, const float *, float *, const PointIndex &);
+hipError_t point_density(hipStream_t, const PointIndex &, int, float *);
+hipError_t loo_multi(hipStream_t, const PointIndex &, const float *, const double *, const double *, int, float *);
+hipError_t range_count(hipStream_t, const PointIndex &, int64_t *);
+hipError_t range_fill(hipStream_t, const PointIndex &, const int64_t *, int32_t *);
+hipError_t valid_cells(hipStream_t, int64_t, const float *, float, uint8_t *);
+hipError_t boundary(hipStream_t, const GridDesc &, const uint8_t *, uint8_t *, float2 *, unsigned long long *);
+hipError_t dist_boundary(hipStream_t, int64_t, const float *, const float *, const float2 *, int64_t, double *);
+hipError_t ec_active(hipStream_t, int64_t, const double *, const float *, float, float *, int *, int *);
+hipError_t ec(hipStream_t, int, const GridDesc &, const MaskRuns &, const float *, const float *, const float *, float,
+              const int *, float *, unsigned long long *);
+hipError_t count_pairs(hipStream_t, const GridDesc &, const MaskRuns &, const PointIndex &, unsigned long long *);
+hipError_t reach(hipStream_t, int, const double *, double *);
+hipError_t gml(hipStream_t, const double *, int64_t, double *);
+hipError_t sumlog(hipStream_t, int64_t, int, const float *, int, double *, int, double *);
+hipError_t abramson(hipStream_t, int64_t, const int *, const float *, const double *, float, float, float *);
+hipError_t unsort_f32(hipStream_t, int64_t, const int *, const float *, float *);
+hipError_t sort_f32(hipStream_t, int64_t, const int *, const float *, float *);
+hipError_t coef_k(hipStream_t, int64_t, const int *, const float *, const float *, double, float *);
+hipError_t fill_f32(hipStream_t, int64_t, float, float *);
+}  // namespace mkde_dev
+
+// ------------------------------------------------------------ errors ----
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                  \
+    do {                                                                                              \
+        hipError_t _e = (expr);                                                                       \
+        if (_e != hipSuccess) return fail(MKDE_E_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+#define CHK(expr)                    \
+    do {                             \
+        int _s = (expr);             \
+        if (_s != MKDE_OK) return _s; \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int ensure(size_t cnt)
+    {
+        if (cnt == 0) cnt = 1;
+        if (p && cnt <= n) return MKDE_OK;
+        release();
+        hipError_t e = hipMalloc((void **)&p, cnt * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(MKDE_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", cnt * sizeof(T), hipGetErrorString(e));
+        }
+        n = cnt;
+        return MKDE_OK;
+    }
+};
+
+// -------------------------------------------------------------- context --
+
+struct RunsHost {
+    std::vector<int64_t> ptr;
+    std::vector<int2> se;
+};
+
+struct mkde_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_density_ms = 0.0f, last_ec_ms = 0.0f, last_points_ms = 0.0f;
+
+    // points, original order (host copies kept for the rule of thumb)
+    int64_t n = 0;
+    std::vector<float> hx, hy, hw;
+    DevBuf<float> x, y, w;
+
+    // cell-binned index
+    BinGrid bg{};
+    int nbins = 0;
+    std::vector<int> hperm;
+    DevBuf<int> perm;           // bin-order position -> original index
+    DevBuf<int> bin_start;      // nbins + 1
+    DevBuf<float> xs, ys;       // bin order
+    DevBuf<PtRec> pts;          // bin order, per-job constants
+    DevBuf<BinInfo> bin_box;
+    DevBuf<double> bin_thr;
+    DevBuf<double> reach;       // 1 value
+    bool pts_dirty = true;
+
+    // per-job arrays, original order
+    DevBuf<float> h, ew;
+    DevBuf<double> dist;
+    bool h_set = false;
+    float h_fixed = 0.0f;       // > 0: every point has this bandwidth
+    bool dist_set = false;
+
+    // edge-correction mask
+    bool mask_set = false;
+    GridDesc mg{};
+    DevBuf<float> mask;
+    DevBuf<uint8_t> valid;
+    DevBuf<int64_t> run_ptr;
+    DevBuf<int2> run_se;
+    DevBuf<uint8_t> bflags;
+    DevBuf<float2> blist;
+    DevBuf<unsigned long long> bcount;
+    int64_t nboundary = -1;
+
+    // tile
+    bool tile_set = false;
+    bool tile_has_mask = false;
+    GridDesc tg{};
+    DevBuf<float> tmask, tout;
+
+    // scratch
+    DevBuf<float> den, den2;
+    DevBuf<int> list;
+    DevBuf<int> icount;
+    DevBuf<unsigned long long> ucount;
+    DevBuf<double> partial, dsum, gval;
+    DevBuf<float> coefk, ewk, den1k;
+    DevBuf<double> ak, tk;
+};
+
+extern "C" const char *mkde_last_error(void) { return g_err.c_str(); }
+
+extern "C" const char *mkde_version(void) { return "mkde 0.1 (gfx950)"; }
+
+static int set_device(mkde_ctx *c)
+{
+    HIPCHK(hipSetDevice(c->device));
+    return MKDE_OK;
+}
+
+extern "C" int mkde_ctx_create(int device, mkde_ctx **out)
+{
+    if (!out) return fail(MKDE_E_ARG, "mkde_ctx_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) return fail(MKDE_E_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= ndev) return fail(MKDE_E_ARG, "device %d out of range [0,%d)", device, ndev);
+    mkde_ctx *c = new (std::nothrow) mkde_ctx();
+    if (!c) return fail(MKDE_E_NOMEM, "out of host memory");
+    c->device = device;
+    e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(MKDE_E_HIP, "context set-up failed: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_ctx_destroy(mkde_ctx *c)
+{
+    if (!c) return MKDE_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_synchronize(mkde_ctx *c)
+{
+    if (!c) return fail(MKDE_E_ARG, "null context");
+    CHK(set_device(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MKDE_OK;
+}
+
+static int copy_h2d(mkde_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return MKDE_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MKDE_OK;
+}
+
+static int copy_d2h(mkde_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return MKDE_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MKDE_OK;
+}
+
+// ---------------------------------------------------------- point index --
+//
+// Uniform bins over the points' bounding box, about 32 points per bin, at most
+// 4M bins; a stable counting sort gives the bin order (ties keep the caller's
+// order, so every downstream summation order is deterministic).
+
+static std::vector<int> build_bins(mkde_ctx *c)
+{
+    const int64_t n = c->n;
+    float x0 = FLT_MAX, x1 = -FLT_MAX, y0 = FLT_MAX, y1 = -FLT_MAX;
+    for (int64_t i = 0; i < n; ++i) {
+        x0 = std::min(x0, c->hx[i]);
+        x1 = std::max(x1, c->hx[i]);
+        y0 = std::min(y0, c->hy[i]);
+        y1 = std::max(y1, c->hy[i]);
+    }
+    if (n == 0) { x0 = y0 = 0.0f; x1 = y1 = 1.0f; }
+    double wx = std::max((double)x1 - x0, 1e-6), wy = std::max((double)y1 - y0, 1e-6);
+    double target = std::max(1.0, (double)n / 32.0);
+    double size = std::sqrt(wx * wy / target);
+    size = std::max(size, std::max(wx, wy) / 4096.0);
+    int nbx, nby;
+    for (;;) {
+        nbx = (int)std::floor(wx / size) + 1;
+        nby = (int)std::floor(wy / size) + 1;
+        if ((int64_t)nbx * nby <= (4 << 20)) break;
+        size *= 1.25;
+    }
+    c->bg = BinGrid{(double)x0, (double)y0, size, nbx, nby};
+    c->nbins = nbx * nby;
+    std::vector<int> key(n), start((size_t)c->nbins + 1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        int bx = (int)std::floor(((double)c->hx[i] - x0) / size);
+        int by = (int)std::floor(((double)c->hy[i] - y0) / size);
+        bx = std::min(std::max(bx, 0), nbx - 1);
+        by = std::min(std::max(by, 0), nby - 1);
+        key[i] = by * nbx + bx;
+        start[key[i] + 1]++;
+    }
+    for (int b = 0; b < c->nbins; ++b) start[b + 1] += start[b];
+    c->hperm.assign(n, 0);
+    std::vector<int> pos(start.begin(), start.end() - 1);
+    for (int64_t i = 0; i < n; ++i) c->hperm[pos[key[i]]++] = (int)i;
+    return start;
+}
+
+extern "C" int mkde_points_upload(mkde_ctx *c, int64_t n, const float *x, const float *y, const float *w)
+{
+    if (!c) return fail(MKDE_E_ARG, "null context");
+    if (n < 0 || (n > 0 && (!x || !y))) return fail(MKDE_E_ARG, "mkde_points_upload: bad arguments");
+    if (n > (int64_t)1 << 30) return fail(MKDE_E_RANGE, "too many points (%lld > 2^30)", (long long)n);
+    CHK(set_device(c));
+    try {
+        c->n = n;
+        c->hx.assign(x, x + n);
+        c->hy.assign(y, y + n);
+        if (w) c->hw.assign(w, w + n);
+        else c->hw.assign(n, 1.0f);
+        for (int64_t i = 0; i < n; ++i)
+            if (!std::isfinite(c->hx[i]) || !std::isfinite(c->hy[i]))
+                return fail(MKDE_E_ARG, "point %lld has a non-finite coordinate", (long long)i);
+        std::vector<int> bs = build_bins(c);
+        std::vector<float> sx(n), sy(n);
+        for (int64_t i = 0; i < n; ++i) {
+            sx[i] = c->hx[c->hperm[i]];
+            sy[i] = c->hy[c->hperm[i]];
+        }
+        CHK(c->x.ensure(n));
+        CHK(c->y.ensure(n));
+        CHK(c->w.ensure(n));
+        CHK(c->xs.ensure(n));
+        CHK(c->ys.ensure(n));
+        CHK(c->perm.ensure(n));
+        CHK(c->bin_start.ensure(c->nbins + 1));
+        CHK(c->pts.ensure(n));
+        CHK(c->bin_box.ensure(c->nbins));
+        CHK(c->bin_thr.ensure(c->nbins));
+        CHK(c->reach.ensure(1));
+        CHK(c->h.ensure(n));
+        CHK(c->ew.ensure(n));
+        CHK(c->dist.ensure(n));
+        CHK(c->den.ensure(n));
+        CHK(copy_h2d(c, c->x.p, c->hx.data(), n * sizeof(float)));
+        CHK(copy_h2d(c, c->y.p, c->hy.data(), n * sizeof(float)));
+        CHK(copy_h2d(c, c->w.p, c->hw.data(), n * sizeof(float)));
+        CHK(copy_h2d(c, c->xs.p, sx.data(), n * sizeof(float)));
+        CHK(copy_h2d(c, c->ys.p, sy.data(), n * sizeof(float)));
+        CHK(copy_h2d(c, c->perm.p, c->hperm.data(), n * sizeof(int)));
+        CHK(copy_h2d(c, c->bin_start.p, bs.data(), bs.size() * sizeof(int)));
+    } catch (const std::bad_alloc &) {
+        return fail(MKDE_E_NOMEM, "out of host memory");
+    }
+    HIPCHK(mkde_dev::fill_f32(c->stream, n, 1.0f, c->ew.p));
+    HIPCHK(mkde_dev::fill_f32(c->stream, n, 0.0f, c->h.p));
+    c->h_set = false;
+    c->h_fixed = 0.0f;
+    c->dist_set = false;
+    c->pts_dirty = true;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MKDE_OK;
+}
+
+extern "C" int mkde_num_points(mkde_ctx *c, int64_t *n)
+{
+    if (!c || !n) return fail(MKDE_E_ARG, "null argument");
+    *n = c->n;
+    return MKDE_OK;
+}
+
+// Rebuild the bin-order point records after bandwidths or EC weights changed.
+static int prep(mkde_ctx *c)
+{
+    if (!c->pts_dirty) return MKDE_OK;
+    HIPCHK(mkde_dev::prep_points(c->stream, c->n, c->perm.p, c->xs.p, c->ys.p, c->w.p, c->h.p, c->ew.p, c->h_fixed,
+                                 c->pts.p));
+    HIPCHK(mkde_dev::bin_stats(c->stream, c->nbins, c->bin_start.p, c->pts.p, c->bin_box.p, c->bin_thr.p));
+    HIPCHK(mkde_dev::reach(c->stream, c->nbins, c->bin_thr.p, c->reach.p));
+    c->pts_dirty = false;
+    return MKDE_OK;
+}
+
+static PointIndex index_of(mkde_ctx *c)
+{
+    PointIndex I;
+    I.pts = c->pts.p;
+    I.bin_start = c->bin_start.p;
+    I.bin_box = c->bin_box.p;
+    I.bin_thr = c->bin_thr.p;
+    I.bg = c->bg;
+    I.reach_p = c->reach.p;
+    I.n = c->n;
+    return I;
+}
+
+static int need_points(mkde_ctx *c)
+{
+    if (!c) return fail(MKDE_E_ARG, "null context");
+    if (c->n <= 0 || !c->x.p) return fail(MKDE_E_STATE, "no points uploaded");
+    return set_device(c);
+}
+
+// ------------------------------------------------------- rule of thumb --
+// MeanCenter (tiled:1563-1578): float sums in point order; StandardDistance2
+// (tiled:1581-1595): double sums of w * Distance2; BandWidth2 (tiled:64-68).
+// Separate roundings on purpose: no contraction into fused multiply-adds.
+#pragma clang fp contract(off)
+static float rule_of_thumb_host(const std::vector<float> &x, const std::vector<float> &y, const std::vector<float> &w)
+{
+    const size_t n = x.size();
+    float sum_x = 0.0f, sum_y = 0.0f, sum_w = 0.0f;
+    for (size_t p = 0; p < n; ++p) {
+        float wp = w[p];
+        sum_w = sum_w + wp;
+        float tx = wp * x[p];
+        sum_x = sum_x + tx;
+        float ty = wp * y[p];
+        sum_y = sum_y + ty;
+    }
+    float mx = sum_x / sum_w, my = sum_y / sum_w;
+    double sum2 = 0.0, sw = 0.0;
+    for (size_t p = 0; p < n; ++p) {
+        float wp = w[p];
+        sw += wp;
+        double dx = (double)(float)(x[p] - mx);
+        double dy = (double)(float)(y[p] - my);
+        double dxx = dx * dx;
+        double dyy = dy * dy;
+        double d2 = dxx + dyy;
+        double t = (double)wp * d2;
+        sum2 = sum2 + t;
+    }
+    double d2 = sum2 / sw;
+    float f = sqrtf(2.0f / (float)(3u * (unsigned)n));
+    double bw2 = (double)f * d2;
+    return sqrtf((float)bw2);
+}
+#pragma clang fp contract(on)
+
+extern "C" int mkde_rule_of_thumb(mkde_ctx *c, float *h0)
+{
+    CHK(need_points(c));
+    if (!h0) return fail(MKDE_E_ARG, "h0 is NULL");
+    *h0 = rule_of_thumb_host(c->hx, c->hy, c->hw);
+    return MKDE_OK;
+}
+
+// ----------------------------------------------------------------- mask --
+
+static void build_runs(const std::vector<float> &v, int64_t ncols, int64_t nrows, float nodata, RunsHost &R)
+{
+    R.ptr.assign(nrows + 1, 0);
+    R.se.clear();
+    for (int64_t r = 0; r < nrows; ++r) {
+        const float *row = v.data() + r * ncols;
+        int64_t c = 0;
+        while (c < ncols) {
+            while (c < ncols && row[c] == nodata) ++c;
+            if (c >= ncols) break;
+            int64_t s = c;
+            while (c < ncols && row[c] != nodata) ++c;
+            R.se.push_back(int2{(int)s, (int)(c - 1)});
+        }
+        R.ptr[r + 1] = (int64_t)R.se.size();
+    }
+}
+
+static int check_grid(const mkde_grid *g)
+{
+    if (!g) return fail(MKDE_E_ARG, "grid is NULL");
+    if (g->ncols <= 0 || g->nrows <= 0) return fail(MKDE_E_ARG, "grid has no cells");
+    if (g->ncols > INT32_MAX - 1 || g->nrows > INT32_MAX - 1)
+        return fail(MKDE_E_RANGE, "grid dimension exceeds 2^31-1");
+    if (!(g->cellsize > 0.0f) || !std::isfinite(g->cellsize)) return fail(MKDE_E_ARG, "cellsize must be > 0");
+    return MKDE_OK;
+}
+
+static GridDesc to_desc(const mkde_grid *g)
+{
+    return GridDesc{g->ncols, g->nrows, g->xll, g->yll, g->cellsize, g->nodata};
+}
+
+extern "C" int mkde_mask_upload(mkde_ctx *c, const mkde_grid *g, const float *values)
+{
+    if (!c) return fail(MKDE_E_ARG, "null context");
+    CHK(check_grid(g));
+    if (!values) return fail(MKDE_E_ARG, "mask values are NULL");
+    CHK(set_device(c));
+    const int64_t nc = g->ncols * g->nrows;
+    try {
+        std::vector<float> v(values, values + nc);
+        RunsHost R;
+        build_runs(v, g->ncols, g->nrows, g->nodata, R);
+        c->mg = to_desc(g);
+        CHK(c->mask.ensure(nc));
+        CHK(c->valid.ensure(nc));
+        CHK(c->bflags.ensure(nc));
+        CHK(c->run_ptr.ensure(R.ptr.size()));
+        CHK(c->run_se.ensure(R.se.size()));
+        CHK(c->bcount.ensure(1));
+        CHK(copy_h2d(c, c->mask.p, values, nc * sizeof(float)));
+        CHK(copy_h2d(c, c->run_ptr.p, R.ptr.data(), R.ptr.size() * sizeof(int64_t)));
+        CHK(copy_h2d(c, c->run_se.p, R.se.data(), R.se.size() * sizeof(int2)));
+    } catch (const std::bad_alloc &) {
+        return fail(MKDE_E_NOMEM, "out of host memory");
+    }
+    HIPCHK(mkde_dev::valid_cells(c->stream, nc, c->mask.p, g->nodata, c->valid.p));
+    c->mask_set = true;
+    c->nboundary = -1;
+    c->dist_set = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MKDE_OK;
+}
+
+static int need_mask(mkde_ctx *c)
+{
+    if (!c->mask_set) return fail(MKDE_E_STATE, "no edge-correction mask uploaded");
+    return MKDE_OK;
+}
+
+static int ensure_boundary(mkde_ctx *c)
+{
+    CHK(need_mask(c));
+    if (c->nboundary >= 0) return MKDE_OK;
+    const int64_t nc = c->mg.ncols * c->mg.nrows;
+    // upper bound of the boundary list: every valid cell
+    CHK(c->blist.ensure(nc));
+    HIPCHK(hipMemsetAsync(c->bcount.p, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(mkde_dev::boundary(c->stream, c->mg, c->valid.p, c->bflags.p, c->blist.p, c->bcount.p));
+    unsigned long long nb = 0;
+    CHK(copy_d2h(c, &nb, c->bcount.p, sizeof nb));
+    c->nboundary = (int64_t)nb;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_mark_boundary(mkde_ctx *c, uint8_t *flags, int64_t *n_boundary)
+{
+    if (!c) return fail(MKDE_E_ARG, "null context");
+    CHK(set_device(c));
+    CHK(ensure_boundary(c));
+    if (flags) CHK(copy_d2h(c, flags, c->bflags.p, (size_t)(c->mg.ncols * c->mg.nrows)));
+    if (n_boundary) *n_boundary = c->nboundary;
+    return MKDE_OK;
+}
+
+static int ensure_dist(mkde_ctx *c)
+{
+    if (c->dist_set) return MKDE_OK;
+    CHK(ensure_boundary(c));
+    HIPCHK(mkde_dev::dist_boundary(c->stream, c->n, c->x.p, c->y.p, c->blist.p, c->nboundary, c->dist.p));
+    c->dist_set = true;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_dist_to_boundary(mkde_ctx *c, double *dist)
+{
+    CHK(need_points(c));
+    CHK(ensure_dist(c));
+    if (dist) CHK(copy_d2h(c, dist, c->dist.p, c->n * sizeof(double)));
+    return MKDE_OK;
+}
+
+extern "C" int mkde_set_dist_to_boundary(mkde_ctx *c, const double *dist)
+{
+    CHK(need_points(c));
+    if (!dist) return fail(MKDE_E_ARG, "dist is NULL");
+    CHK(copy_h2d(c, c->dist.p, dist, c->n * sizeof(double)));
+    c->dist_set = true;
+    return MKDE_OK;
+}
+
+// ------------------------------------------------------------ bandwidths --
+
+extern "C" int mkde_set_bandwidth_fixed(mkde_ctx *c, float h)
+{
+    CHK(need_points(c));
+    if (!std::isfinite(h)) return fail(MKDE_E_ARG, "bandwidth must be finite");
+    HIPCHK(mkde_dev::fill_f32(c->stream, c->n, h, c->h.p));
+    c->h_fixed = h > 0.0f ? h : 0.0f;
+    c->h_set = true;
+    c->pts_dirty = true;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_set_bandwidths(mkde_ctx *c, const float *h)
+{
+    CHK(need_points(c));
+    if (!h) return fail(MKDE_E_ARG, "h is NULL");
+    CHK(copy_h2d(c, c->h.p, h, c->n * sizeof(float)));
+    c->h_fixed = 0.0f;
+    c->h_set = true;
+    c->pts_dirty = true;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_get_bandwidths(mkde_ctx *c, float *h)
+{
+    CHK(need_points(c));
+    if (!h) return fail(MKDE_E_ARG, "h is NULL");
+    return copy_d2h(c, h, c->h.p, c->n * sizeof(float));
+}
+
+static int need_h(mkde_ctx *c)
+{
+    if (!c->h_set) return fail(MKDE_E_STATE, "bandwidths are not set");
+    return MKDE_OK;
+}
+
+// ------------------------------------------------------- edge correction --
+
+// EC weights for bandwidth array h (original order) or the fixed value hf > 0,
+// written to ew_out (original order).
+static int run_ec(mkde_ctx *c, const float *h, float hf, float *ew_out, unsigned long long *n_eval)
+{
+    CHK(ensure_dist(c));
+    CHK(c->list.ensure(c->n));
+    CHK(c->icount.ensure(1));
+    CHK(c->ucount.ensure(1));
+    HIPCHK(hipMemsetAsync(c->icount.p, 0, sizeof(int), c->stream));
+    HIPCHK(hipMemsetAsync(c->ucount.p, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(mkde_dev::ec_active(c->stream, c->n, c->dist.p, h, hf, ew_out, c->list.p, c->icount.p));
+    int nact = 0;
+    CHK(copy_d2h(c, &nact, c->icount.p, sizeof nact));
+    MaskRuns R{c->run_ptr.p, c->run_se.p};
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(mkde_dev::ec(c->stream, nact, c->mg, R, c->x.p, c->y.p, h, hf, c->list.p, ew_out, c->ucount.p));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    if (n_eval) CHK(copy_d2h(c, n_eval, c->ucount.p, sizeof *n_eval));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&c->last_ec_ms, c->ev0, c->ev1));
+    return MKDE_OK;
+}
+
+extern "C" int mkde_ec_weights(mkde_ctx *c, float *ew, int64_t *n_eval)
+{
+    CHK(need_points(c));
+    CHK(need_mask(c));
+    CHK(need_h(c));
+    unsigned long long ne = 0;
+    CHK(run_ec(c, c->h.p, c->h_fixed, c->ew.p, &ne));
+    c->pts_dirty = true;
+    if (n_eval) *n_eval = (int64_t)ne;
+    if (ew) CHK(copy_d2h(c, ew, c->ew.p, c->n * sizeof(float)));
+    return MKDE_OK;
+}
+
+extern "C" int mkde_set_ec_weights(mkde_ctx *c, const float *ew)
+{
+    CHK(need_points(c));
+    if (ew) CHK(copy_h2d(c, c->ew.p, ew, c->n * sizeof(float)));
+    else HIPCHK(mkde_dev::fill_f32(c->stream, c->n, 1.0f, c->ew.p));
+    c->pts_dirty = true;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_get_ec_weights(mkde_ctx *c, float *ew)
+{
+    CHK(need_points(c));
+    if (!ew) return fail(MKDE_E_ARG, "ew is NULL");
+    return copy_d2h(c, ew, c->ew.p, c->n * sizeof(float));
+}
+
+// -------------------------------------------------------- range search --
+
+extern "C" int mkde_range_candidates(mkde_ctx *c, int64_t *offsets, int32_t *idx, int64_t capacity, int64_t *total)
+{
+    CHK(need_points(c));
+    CHK(need_h(c));
+    CHK(prep(c));
+    const int64_t n = c->n;
+    DevBuf<int64_t> counts;
+    CHK(counts.ensure(n));
+    PointIndex I = index_of(c);
+    HIPCHK(mkde_dev::range_count(c->stream, I, counts.p));
+    std::vector<int64_t> hc(n);
+    CHK(copy_d2h(c, hc.data(), counts.p, n * sizeof(int64_t)));
+    std::vector<int64_t> soff(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i) soff[i + 1] = soff[i] + hc[i];
+    const int64_t tot = soff[n];
+    if (total) *total = tot;
+    if (offsets) {
+        // offsets in the caller's point order
+        std::vector<int64_t> cnt_orig(n);
+        for (int64_t i = 0; i < n; ++i) cnt_orig[c->hperm[i]] = hc[i];
+        offsets[0] = 0;
+        for (int64_t i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + cnt_orig[i];
+    }
+    if (!idx) return MKDE_OK;
+    if (!offsets) return fail(MKDE_E_ARG, "offsets required with idx");
+    if (capacity < tot) return fail(MKDE_E_RANGE, "capacity %lld < %lld candidates", (long long)capacity, (long long)tot);
+    DevBuf<int64_t> doff;
+    DevBuf<int32_t> didx;
+    CHK(doff.ensure(n + 1));
+    CHK(didx.ensure(tot));
+    CHK(copy_h2d(c, doff.p, soff.data(), (n + 1) * sizeof(int64_t)));
+    HIPCHK(mkde_dev::range_fill(c->stream, I, doff.p, didx.p));
+    std::vector<int32_t> hs(tot);
+    CHK(copy_d2h(c, hs.data(), didx.p, tot * sizeof(int32_t)));
+    for (int64_t i = 0; i < n; ++i) {
+        const int o = c->hperm[i];
+        int32_t *dst = idx + offsets[o];
+        for (int64_t k = soff[i]; k < soff[i + 1]; ++k) *dst++ = c->hperm[hs[k]];
+        std::sort(idx + offsets[o], idx + offsets[o + 1]);
+    }
+    return MKDE_OK;
+}
+
+// ------------------------------------------------- densities at points --
+
+static int density_points_sorted(mkde_ctx *c, int loo, float *den_sorted)
+{
+    CHK(prep(c));
+    PointIndex I = index_of(c);
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(mkde_dev::point_density(c->stream, I, loo, den_sorted));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    return MKDE_OK;
+}
+
+extern "C" int mkde_density_at_points(mkde_ctx *c, int loo, float *den)
+{
+    CHK(need_points(c));
+    CHK(need_h(c));
+    CHK(c->den2.ensure(c->n));
+    CHK(density_points_sorted(c, loo ? 1 : 0, c->den.p));
+    HIPCHK(mkde_dev::unsort_f32(c->stream, c->n, c->perm.p, c->den.p, c->den2.p));
+    if (den) CHK(copy_d2h(c, den, c->den2.p, c->n * sizeof(float)));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&c->last_points_ms, c->ev0, c->ev1));
+    return MKDE_OK;
+}
+
+// FP64 sum of log(max(den, 1e-14f)) (clamp) or log(den) over K rows of n.
+static int sum_logs(mkde_ctx *c, const float *den, int K, int clamp, double *out_dev)
+{
+    const int nb = (int)std::min<int64_t>(1024, (c->n + 255) / 256);
+    CHK(c->partial.ensure((size_t)nb * K));
+    HIPCHK(mkde_dev::sumlog(c->stream, c->n, K, den, clamp, c->partial.p, nb, out_dev));
+    return MKDE_OK;
+}
+
+// --------------------------------------------------------------- LOO ----
+
+extern "C" int mkde_loo_loglik(mkde_ctx *c, int K, const float *h_cand, int ec_on, double *logL)
+{
+    CHK(need_points(c));
+    if (K <= 0 || K > 1024 || !h_cand || !logL) return fail(MKDE_E_ARG, "mkde_loo_loglik: bad arguments");
+    if (ec_on) CHK(need_mask(c));
+    for (int k = 0; k < K; ++k)
+        if (!(h_cand[k] > 0.0f) || !std::isfinite(h_cand[k])) return fail(MKDE_E_ARG, "candidate %d is not > 0", k);
+    const int64_t n = c->n;
+    // candidates in ascending order of h2f
+    std::vector<int> ord(K);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return h_cand[a] < h_cand[b]; });
+    // positions for the index: prep with the largest candidate so that the bin
+    // thresholds cover every candidate
+    float hmax = h_cand[ord[K - 1]];
+    const float saved_hf = c->h_fixed;
+    const bool saved_hset = c->h_set;
+    DevBuf<float> hsave;
+    CHK(hsave.ensure(n));
+    HIPCHK(hipMemcpyAsync(hsave.p, c->h.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    c->h_fixed = hmax;
+    c->pts_dirty = true;
+    CHK(prep(c));
+    PointIndex I = index_of(c);
+    const int G = std::min(K, MKDE_LM_K);
+    CHK(c->coefk.ensure((size_t)G * n));
+    CHK(c->den1k.ensure((size_t)G * n));
+    CHK(c->ewk.ensure(n));
+    CHK(c->ak.ensure(G));
+    CHK(c->tk.ensure(G));
+    CHK(c->dsum.ensure(K));
+    std::vector<double> res(K);
+    for (int g0 = 0; g0 < K; g0 += MKDE_LM_K) {
+        const int kg = std::min(MKDE_LM_K, K - g0);
+        std::vector<double> a(kg), t(kg);
+        for (int j = 0; j < kg; ++j) {
+            const float hk = h_cand[ord[g0 + j]];
+            const double h2 = (double)(float)(hk * hk);
+            a[j] = -1.4426950408889634073599 / (2.0 * h2);
+            t[j] = 9.0 * h2;
+            if (ec_on) CHK(run_ec(c, nullptr, hk, c->ewk.p, nullptr));
+            else HIPCHK(mkde_dev::fill_f32(c->stream, n, 1.0f, c->ewk.p));
+            HIPCHK(mkde_dev::coef_k(c->stream, n, c->perm.p, c->w.p, c->ewk.p, h2, c->coefk.p + (size_t)j * n));
+        }
+        CHK(copy_h2d(c, c->ak.p, a.data(), kg * sizeof(double)));
+        CHK(copy_h2d(c, c->tk.p, t.data(), kg * sizeof(double)));
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        HIPCHK(mkde_dev::loo_multi(c->stream, I, c->coefk.p, c->ak.p, c->tk.p, kg, c->den1k.p));
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        CHK(sum_logs(c, c->den1k.p, kg, 1, c->dsum.p));
+        std::vector<double> part(kg);
+        CHK(copy_d2h(c, part.data(), c->dsum.p, kg * sizeof(double)));
+        HIPCHK(hipEventElapsedTime(&c->last_points_ms, c->ev0, c->ev1));
+        for (int j = 0; j < kg; ++j) res[ord[g0 + j]] = part[j];
+    }
+    for (int k = 0; k < K; ++k) logL[k] = res[k];
+    // restore bandwidths
+    HIPCHK(hipMemcpyAsync(c->h.p, hsave.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->h_fixed = saved_hf;
+    c->h_set = saved_hset;
+    c->pts_dirty = true;
+    return MKDE_OK;
+}
+
+// Abramson pipeline on the context: EC(h0) if ec_on, pilot at h0, g, h_i.
+// Leaves h = h_i and ew = EC(h0) (or 1).
+static int abramson_core(mkde_ctx *c, float h0, float alpha, int ec_on)
+{
+    CHK(mkde_set_bandwidth_fixed(c, h0));
+    if (ec_on) {
+        CHK(run_ec(c, nullptr, h0, c->ew.p, nullptr));
+    } else {
+        HIPCHK(mkde_dev::fill_f32(c->stream, c->n, 1.0f, c->ew.p));
+    }
+    c->pts_dirty = true;
+    CHK(density_points_sorted(c, 0, c->den.p));
+    CHK(c->dsum.ensure(1));
+    CHK(c->gval.ensure(1));
+    CHK(sum_logs(c, c->den.p, 1, 0, c->dsum.p));
+    HIPCHK(mkde_dev::gml(c->stream, c->dsum.p, c->n, c->gval.p));
+    HIPCHK(mkde_dev::abramson(c->stream, c->n, c->perm.p, c->den.p, c->gval.p, h0, alpha, c->h.p));
+    c->h_fixed = 0.0f;
+    c->h_set = true;
+    c->pts_dirty = true;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_abramson(mkde_ctx *c, float h0, float alpha, int ec_on, float *h_out, float *den0_out,
+                             double *g_out)
+{
+    CHK(need_points(c));
+    if (!(h0 > 0.0f) || !std::isfinite(h0) || !std::isfinite(alpha)) return fail(MKDE_E_ARG, "bad h0/alpha");
+    if (ec_on) CHK(need_mask(c));
+    CHK(abramson_core(c, h0, alpha, ec_on));
+    if (h_out) CHK(copy_d2h(c, h_out, c->h.p, c->n * sizeof(float)));
+    if (den0_out) {
+        CHK(c->den2.ensure(c->n));
+        HIPCHK(mkde_dev::unsort_f32(c->stream, c->n, c->perm.p, c->den.p, c->den2.p));
+        CHK(copy_d2h(c, den0_out, c->den2.p, c->n * sizeof(float)));
+    }
+    if (g_out) CHK(copy_d2h(c, g_out, c->gval.p, sizeof(double)));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MKDE_OK;
+}
+
+extern "C" int mkde_loo_loglik_adaptive(mkde_ctx *c, int K, const float *h, const float *alpha, int ec_on,
+                                        double *logL)
+{
+    CHK(need_points(c));
+    if (K <= 0 || !h || !alpha || !logL) return fail(MKDE_E_ARG, "mkde_loo_loglik_adaptive: bad arguments");
+    if (ec_on) CHK(need_mask(c));
+    const int64_t n = c->n;
+    DevBuf<float> hsave, ewsave;
+    CHK(hsave.ensure(n));
+    CHK(ewsave.ensure(n));
+    HIPCHK(hipMemcpyAsync(hsave.p, c->h.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(ewsave.p, c->ew.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    const float saved_hf = c->h_fixed;
+    const bool saved_hset = c->h_set;
+    CHK(c->dsum.ensure(1));
+    for (int k = 0; k < K; ++k) {
+        if (!(h[k] > 0.0f) || !std::isfinite(h[k]) || !std::isfinite(alpha[k])) {
+            logL[k] = -INFINITY;
+            continue;
+        }
+        CHK(abramson_core(c, h[k], alpha[k], ec_on));
+        if (ec_on) CHK(run_ec(c, c->h.p, 0.0f, c->ew.p, nullptr));
+        c->pts_dirty = true;
+        CHK(density_points_sorted(c, 1, c->den.p));
+        CHK(sum_logs(c, c->den.p, 1, 1, c->dsum.p));
+        CHK(copy_d2h(c, &logL[k], c->dsum.p, sizeof(double)));
+    }
+    HIPCHK(hipMemcpyAsync(c->h.p, hsave.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->ew.p, ewsave.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->h_fixed = saved_hf;
+    c->h_set = saved_hset;
+    c->pts_dirty = true;
+    return MKDE_OK;
+}
+
+// ------------------------------------------------------------ optimisers --
+// MLE_FixedBandWidth (tiled:3044-3093) with the float control flow of the
+// reference; the two probes of an iteration are one batched LOO call and the
+// comparison is on the FP64 log-likelihoods.
+#pragma clang fp contract(off)
+extern "C" int mkde_mle_fixed_bandwidth(mkde_ctx *c, float h, int ec_on, float *hopt, int *n_iter)
+{
+    CHK(need_points(c));
+    if (!hopt || !(h > 0.0f)) return fail(MKDE_E_ARG, "mkde_mle_fixed_bandwidth: bad arguments");
+    float hA = h / 20;
+    float hD = 2 * h;
+    float width = hD - hA;
+    float epsilon = width / 100;
+    float factor = 1 + sqrtf(5.0f);
+    int iteration = 0;
+    while (width > epsilon && iteration < 30) {
+        float hB = hA + width / factor;
+        float hC = hD - width / factor;
+        float hs[2] = {hB, hC};
+        double L[2];
+        CHK(mkde_loo_loglik(c, 2, hs, ec_on, L));
+        if (L[0] > L[1]) hD = hC;
+        else hA = hB;
+        width = hD - hA;
+        iteration += 1;
+    }
+    *hopt = (hA + hD) / 2;
+    if (n_iter) *n_iter = iteration;
+    return MKDE_OK;
+}
+
+// hooke_jeeves + hj_likelihood (tiled:3677-3790), bookkeeping as in the
+// reference (likelihoods stored as float, running maximum).
+extern "C" int mkde_hooke_jeeves(mkde_ctx *c, float h0, float alpha0, float stepH, float stepA, int ec_on, float *out3,
+                                 int *n_iter)
+{
+    CHK(need_points(c));
+    if (!out3) return fail(MKDE_E_ARG, "out3 is NULL");
+    float Ls[5] = {0, 0, 0, 0, 0};
+    auto probe = [&](int last) -> int {
+        float hh[5], aa[5];
+        int which[5], m = 0;
+        if (last == -1) { hh[m] = h0; aa[m] = alpha0; which[m++] = 0; }
+        if (last != 2) { hh[m] = h0 - stepH; aa[m] = alpha0; which[m++] = 1; }
+        if (last != 1) { hh[m] = h0 + stepH; aa[m] = alpha0; which[m++] = 2; }
+        if (last != 4) { hh[m] = h0; aa[m] = alpha0 + stepA; which[m++] = 3; }
+        if (last != 3) { hh[m] = h0; aa[m] = alpha0 - stepA; which[m++] = 4; }
+        double L[5];
+        int s = mkde_loo_loglik_adaptive(c, m, hh, aa, ec_on, L);
+        if (s != MKDE_OK) return s;
+        for (int i = 0; i < m; ++i) Ls[which[i]] = (float)L[i];
+        return MKDE_OK;
+    };
+    CHK(probe(-1));
+    float Lmax = Ls[0];
+    float s = stepH / 10;
+    float a = stepA / 10;
+    int iteration = 0;
+    while ((stepH > s || stepA > a) && (h0 - stepH > 0) && iteration <= 30) {
+        int dmax = 0;
+        for (int i = 0; i < 5; i++) {
+            if (Ls[i] > Lmax) {
+                Lmax = Ls[i];
+                dmax = i;
+            }
+        }
+        if (dmax == 0) {
+            stepH = stepH / 2;
+            stepA = stepA / 2;
+        } else {
+            if (dmax == 1) { h0 = h0 - stepH; Ls[2] = Ls[0]; Ls[0] = Ls[1]; }
+            if (dmax == 2) { h0 = h0 + stepH; Ls[1] = Ls[0]; Ls[0] = Ls[2]; }
+            if (dmax == 3) { alpha0 = alpha0 + stepA; Ls[3] = Ls[0]; Ls[0] = Ls[4]; }
+            if (dmax == 4) { alpha0 = alpha0 - stepA; Ls[3] = Ls[0]; Ls[0] = Ls[4]; }
+        }
+        CHK(probe(dmax));
+        iteration++;
+    }
+    out3[0] = h0;
+    out3[1] = alpha0;
+    out3[2] = Lmax;
+    if (n_iter) *n_iter = iteration;
+    return MKDE_OK;
+}
+#pragma clang fp contract(on)
+
+// ------------------------------------------------------------ density ----
+
+extern "C" int mkde_tile_upload(mkde_ctx *c, const mkde_grid *tile, const float *mask)
+{
+    if (!c) return fail(MKDE_E_ARG, "null context");
+    CHK(check_grid(tile));
+    CHK(set_device(c));
+    const int64_t nc = tile->ncols * tile->nrows;
+    c->tg = to_desc(tile);
+    CHK(c->tout.ensure(nc));
+    c->tile_has_mask = mask != nullptr;
+    if (mask) {
+        CHK(c->tmask.ensure(nc));
+        CHK(copy_h2d(c, c->tmask.p, mask, nc * sizeof(float)));
+    }
+    c->tile_set = true;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_density_tile_resident(mkde_ctx *c)
+{
+    CHK(need_points(c));
+    CHK(need_h(c));
+    if (!c->tile_set) return fail(MKDE_E_STATE, "no tile uploaded");
+    CHK(prep(c));
+    PointIndex I = index_of(c);
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(mkde_dev::density_tile(c->stream, c->tg, c->tile_has_mask ? c->tmask.p : nullptr, c->tout.p, I));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    return MKDE_OK;
+}
+
+extern "C" int mkde_tile_download(mkde_ctx *c, float *out)
+{
+    if (!c || !out) return fail(MKDE_E_ARG, "null argument");
+    if (!c->tile_set) return fail(MKDE_E_STATE, "no tile uploaded");
+    CHK(set_device(c));
+    CHK(copy_d2h(c, out, c->tout.p, (size_t)(c->tg.ncols * c->tg.nrows) * sizeof(float)));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&c->last_density_ms, c->ev0, c->ev1));
+    return MKDE_OK;
+}
+
+extern "C" int mkde_density_tile(mkde_ctx *c, const mkde_grid *tile, const float *mask, float *out)
+{
+    CHK(mkde_tile_upload(c, tile, mask));
+    CHK(mkde_density_tile_resident(c));
+    if (out) return mkde_tile_download(c, out);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipEventElapsedTime(&c->last_density_ms, c->ev0, c->ev1));
+    return MKDE_OK;
+}
+
+extern "C" int mkde_count_density_pairs(mkde_ctx *c, const mkde_grid *tile, const float *mask, int64_t *n_pairs)
+{
+    CHK(need_points(c));
+    CHK(need_h(c));
+    CHK(check_grid(tile));
+    if (!n_pairs) return fail(MKDE_E_ARG, "n_pairs is NULL");
+    const int64_t nc = tile->ncols * tile->nrows;
+    RunsHost R;
+    try {
+        if (mask) {
+            std::vector<float> v(mask, mask + nc);
+            build_runs(v, tile->ncols, tile->nrows, tile->nodata, R);
+        } else {
+            R.ptr.resize(tile->nrows + 1);
+            R.se.resize(tile->nrows);
+            for (int64_t r = 0; r <= tile->nrows; ++r) R.ptr[r] = r;
+            for (int64_t r = 0; r < tile->nrows; ++r) R.se[r] = int2{0, (int)(tile->ncols - 1)};
+        }
+    } catch (const std::bad_alloc &) {
+        return fail(MKDE_E_NOMEM, "out of host memory");
+    }
+    DevBuf<int64_t> rp;
+    DevBuf<int2> rs;
+    CHK(rp.ensure(R.ptr.size()));
+    CHK(rs.ensure(R.se.size()));
+    CHK(copy_h2d(c, rp.p, R.ptr.data(), R.ptr.size() * sizeof(int64_t)));
+    CHK(copy_h2d(c, rs.p, R.se.data(), R.se.size() * sizeof(int2)));
+    CHK(prep(c));
+    CHK(c->ucount.ensure(1));
+    HIPCHK(hipMemsetAsync(c->ucount.p, 0, sizeof(unsigned long long), c->stream));
+    MaskRuns MR{rp.p, rs.p};
+    PointIndex I = index_of(c);
+    GridDesc g = to_desc(tile);
+    HIPCHK(mkde_dev::count_pairs(c->stream, g, MR, I, c->ucount.p));
+    unsigned long long t = 0;
+    CHK(copy_d2h(c, &t, c->ucount.p, sizeof t));
+    *n_pairs = (int64_t)t;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_last_kernel_ms(mkde_ctx *c, const char *which, double *ms)
+{
+    if (!c || !which || !ms) return fail(MKDE_E_ARG, "null argument");
+    if (!strcmp(which, "density")) *ms = c->last_density_ms;
+    else if (!strcmp(which, "ec")) *ms = c->last_ec_ms;
+    else if (!strcmp(which, "points")) *ms = c->last_points_ms;
+    else return fail(MKDE_E_ARG, "unknown kernel '%s'", which);
+    return MKDE_OK;
+}

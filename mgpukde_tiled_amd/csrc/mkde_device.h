@@ -1,0 +1,58 @@
+// mkde_device.h -- types shared by the HIP kernels (mkde_kernels.hip) and the
+// host library (mkde_capi.cpp).  Plain structs only: no torch, no STL.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Utilities.h:8 -- the reference's TWO_PI is a float literal.
+#define MKDE_TWO_PI_F 6.2831853071795862f
+#define MKDE_LOG2E 1.4426950408889634073599
+// Utilities.h:19 -- EPSILONDENSITY (a float literal).
+#define MKDE_EPS_DEN 0.00000000000001f
+// candidate bandwidths per batched leave-one-out pass
+#define MKDE_LM_K 16
+
+// Raster geometry (AsciiRaster.h:9-33 geometry fields).
+struct GridDesc {
+    int64_t ncols, nrows;
+    float xll, yll, cs, nodata;
+};
+
+// Uniform world-space bin grid over the points' bounding box.  Bin (bx, by)
+// covers [x0 + bx*size, x0 + (bx+1)*size) x [y0 + by*size, ...).
+struct BinGrid {
+    double x0, y0, size;
+    int nbx, nby;
+};
+
+// One sample point in bin order, with the per-job derived constants of the
+// Gaussian kernel: coef = w*ew / (2*pi*h2f), a = -log2(e) / (2*h2f),
+// thr = 9*h2f, where h2f = (double)fl32(h*h).  A point with h <= 0 has
+// thr = -1 (it can never be inside the cut-off) and coef = 0.
+struct __attribute__((aligned(32))) PtRec {
+    float x, y, coef, pad;
+    double a, thr;
+};
+
+// Per-bin summary used for culling: bounding box of the bin's points and the
+// largest cut-off threshold 9*h2f among them.
+struct BinInfo {
+    float x0, y0, x1, y1;
+};
+
+struct PointIndex {
+    const PtRec *pts;       // n, bin order
+    const int *bin_start;   // nbins + 1
+    const BinInfo *bin_box; // nbins
+    const double *bin_thr;  // nbins
+    BinGrid bg;
+    const double *reach_p;  // device scalar: max over bins of sqrt(bin_thr), world units
+    int64_t n;
+};
+
+// Per-row runs of valid cells of a mask (CSR): runs of row r are
+// run_se[run_ptr[r] .. run_ptr[r+1]), each (start, end) inclusive, ascending.
+struct MaskRuns {
+    const int64_t *run_ptr;
+    const int2 *run_se;
+};

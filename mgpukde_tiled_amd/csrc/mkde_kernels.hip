@@ -1,0 +1,1347 @@
+// mkde_kernels.hip -- hand-written HIP kernels for gfx950 (CDNA4, wave64).
+//
+// Hot path of the reference (Guiming/mGPUKDE_tiled) re-designed for MI355X:
+//   k_density_tile    per-cell gather density over 64x64-cell blocks
+//                     (replaces KernelDesityEstimation_pPoints,
+//                     kde_kernel_kdtr.cu:461-574, and the CPU loop tiled:707-733)
+//   k_ec              per-point edge-correction integral, O(R) per point
+//                     (replaces CalcEdgeCorrectionWeights kde_kernel_kdtr.cu:117-265
+//                     and EdgeCorrectionWeightsExact tiled:2694-2807)
+//   k_point_density   density at points over the cell-binned index, pilot and
+//                     leave-one-out forms (replaces DensityAtPointsKdtr
+//                     kde_kernel_kdtr.cu:681-953 and the CPU scatter tiled:3312-3347)
+//   k_loo_multi       leave-one-out densities of up to 16 fixed bandwidths in one
+//                     pass over point pairs (batched LogLikelihood, tiled:3214-3368)
+//   k_boundary, k_dist_boundary   MarkBoundary / CalcDist2Boundary
+//                     (kde_kernel_kdtr.cu:1209-1353, tiled:3993-4131)
+//   reductions        FP64 sums of logs (replaces ReductionSum kde_kernel_kdtr.cu:1045-1111)
+//
+// Numerics shared by every kernel (the reference CPU path's rules):
+//   cell centre  cx = fl32(xll + (c+0.5)*cs), cy = fl32(yll + (nrows-0.5-r)*cs)
+//                evaluated in double (Utilities.h:13-14)
+//   cut-off      fl64(dx^2 + dy^2) < 9*h2f with dx = fl32(cx - px), h2f = fl32(h*h)
+//                -- evaluated exactly (both squares are exact in double)
+//   kernel       K = exp(-d^2/(2 h2f)) / (2*pi*h2f), computed as
+//                exp2(fl32(d^2 * a)) with a = -log2(e)/(2 h2f); the density and
+//                edge-correction integrands use the separable form
+//                exp2(dx^2 a) * exp2(dy^2 a) over raster cells.
+//   sums         FP32 with compensation (Kahan folds of short FP32 partial sums,
+//                float-float prefix sums); FP64 only for log-likelihood sums.
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "mkde_device.h"
+
+#define WAVE 64
+
+// ------------------------------------------------------------------ helpers --
+
+__device__ __forceinline__ float cell_x(double xll, double cs, int64_t c)
+{
+    return (float)__dadd_rn(xll, __dmul_rn((double)c + 0.5, cs));
+}
+
+// nrm05 = (double)nrows - 0.5 (ROW_TO_YCOORD evaluates (NROWS - 0.5 - ROW)).
+__device__ __forceinline__ float cell_y(double yll, double cs, double nrm05, int64_t r)
+{
+    return (float)__dadd_rn(yll, __dmul_rn(nrm05 - (double)r, cs));
+}
+
+// fl64(a^2 + b^2) for float a, b: both squares are exact in double.
+__device__ __forceinline__ double sq2(float a, float b)
+{
+    double da = a, db = b;
+    return __dadd_rn(__dmul_rn(da, da), __dmul_rn(db, db));
+}
+
+__device__ __forceinline__ double sqd(float a)
+{
+    double da = a;
+    return __dmul_rn(da, da);
+}
+
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// Distance from v to the interval [lo, hi] along one axis as fl32(lo - v) or
+// fl32(v - hi); 0 inside.  Because fl32 subtraction is monotone, for every u in
+// [lo, hi] |fl32(u - v)| >= this value: it is an exact lower bound of the
+// reference's |dx| over the interval.
+__device__ __forceinline__ float gap1(float v, float lo, float hi)
+{
+    return v < lo ? (lo - v) : (v > hi ? (v - hi) : 0.0f);
+}
+
+// Largest |fl32(u - v)| over u in [lo, hi]: attained at an end point.
+__device__ __forceinline__ float far1(float v, float lo, float hi)
+{
+    float a = fabsf(lo - v), b = fabsf(hi - v);
+    return a > b ? a : b;
+}
+
+// Kahan fold of a short FP32 partial sum into (sum, comp).
+__device__ __forceinline__ void kahan_fold(float part, float &sum, float &comp)
+{
+    float y = part - comp;
+    float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+// float-float arithmetic (compensated FP32).
+struct ff {
+    float hi, lo;
+};
+
+__device__ __forceinline__ ff ff_add(ff a, ff b)
+{
+    float s = a.hi + b.hi;
+    float bb = s - a.hi;
+    float e = (a.hi - (s - bb)) + (b.hi - bb);
+    e += a.lo + b.lo;
+    float r = s + e;
+    return ff{r, e - (r - s)};
+}
+
+__device__ __forceinline__ ff ff_add_f(ff a, float b)
+{
+    float s = a.hi + b;
+    float bb = s - a.hi;
+    float e = (a.hi - (s - bb)) + (b - bb);
+    e += a.lo;
+    float r = s + e;
+    return ff{r, e - (r - s)};
+}
+
+__device__ __forceinline__ ff ff_neg(ff a) { return ff{-a.hi, -a.lo}; }
+
+__device__ __forceinline__ ff ff_mul_f(ff a, float b)
+{
+    float p = a.hi * b;
+    float e = __builtin_fmaf(a.hi, b, -p);
+    e = __builtin_fmaf(a.lo, b, e);
+    float r = p + e;
+    return ff{r, e - (r - p)};
+}
+
+__device__ __forceinline__ ff ff_shfl_up(ff a, int d)
+{
+    return ff{__shfl_up(a.hi, d, WAVE), __shfl_up(a.lo, d, WAVE)};
+}
+
+__device__ __forceinline__ ff ff_shfl_xor(ff a, int d)
+{
+    return ff{__shfl_xor(a.hi, d, WAVE), __shfl_xor(a.lo, d, WAVE)};
+}
+
+// Order-preserving compaction across a 256-thread block.  Returns this
+// thread's slot among the flagged threads; *total gets the flagged count.
+__device__ __forceinline__ int block_compact256(bool flag, int *s_wcnt, int *total)
+{
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+    uint64_t m = __ballot(flag);
+    int rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wcnt[wave] = __popcll(m);
+    __syncthreads();
+    int off = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) off += (w < wave) ? s_wcnt[w] : 0;
+    *total = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
+    __syncthreads();
+    return off + rank;
+}
+
+__device__ __forceinline__ int clampi(int64_t v, int64_t lo, int64_t hi)
+{
+    return (int)(v < lo ? lo : (v > hi ? hi : v));
+}
+
+// Candidate-bin range of a query rectangle [qx0,qx1]x[qy0,qy1] grown by reach.
+__device__ __forceinline__ void bin_range(const BinGrid &bg, double qx0, double qx1, double qy0, double qy1,
+                                          double reach, int &bxa, int &bxb, int &bya, int &byb)
+{
+    bxa = clampi((int64_t)floor((qx0 - reach - bg.x0) / bg.size), 0, bg.nbx - 1);
+    bxb = clampi((int64_t)floor((qx1 + reach - bg.x0) / bg.size), 0, bg.nbx - 1);
+    bya = clampi((int64_t)floor((qy0 - reach - bg.y0) / bg.size), 0, bg.nby - 1);
+    byb = clampi((int64_t)floor((qy1 + reach - bg.y0) / bg.size), 0, bg.nby - 1);
+}
+
+// Exact lower bound of the cut-off distance between any point of bin box B and
+// any position in rectangle Q (both in float coordinates).
+__device__ __forceinline__ double box_gap2(const BinInfo &B, float qx0, float qx1, float qy0, float qy1)
+{
+    float gx = B.x1 < qx0 ? (qx0 - B.x1) : (B.x0 > qx1 ? (B.x0 - qx1) : 0.0f);
+    float gy = B.y1 < qy0 ? (qy0 - B.y1) : (B.y0 > qy1 ? (B.y0 - qy1) : 0.0f);
+    return sq2(gx, gy);
+}
+
+// ============================================================ point index ====
+
+// Per-job point records in bin order: gather h and ew through the permutation
+// and derive the kernel constants.
+__global__ void k_prep_points(int64_t n, const int *perm, const float *xs, const float *ys, const float *w_orig,
+                              const float *h_orig, const float *ew_orig, float h_fixed, PtRec *out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int o = perm[i];
+    float h = h_fixed > 0.0f ? h_fixed : (h_orig ? h_orig[o] : 0.0f);
+    float w = w_orig[o];
+    float ew = ew_orig[o];
+    double h2 = (double)(float)(h * h);
+    PtRec r;
+    r.x = xs[i];
+    r.y = ys[i];
+    r.pad = 0.0f;
+    if (h2 > 0.0 && h2 < 1e300) {
+        r.coef = (float)((double)w * (double)ew / (h2 * (double)MKDE_TWO_PI_F));
+        r.a = -MKDE_LOG2E / (2.0 * h2);
+        r.thr = 9.0 * h2;
+    } else {
+        r.coef = 0.0f;
+        r.a = 0.0;
+        r.thr = -1.0;
+    }
+    out[i] = r;
+}
+
+// Per-bin bounding box of the points and largest threshold.
+__global__ void k_bin_stats(int nbins, const int *bin_start, const PtRec *pts, BinInfo *box, double *thr)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbins) return;
+    int s = bin_start[b], e = bin_start[b + 1];
+    float x0 = FLT_MAX, y0 = FLT_MAX, x1 = -FLT_MAX, y1 = -FLT_MAX;
+    double t = -1.0;
+    for (int i = s; i < e; ++i) {
+        PtRec p = pts[i];
+        x0 = fminf(x0, p.x);
+        x1 = fmaxf(x1, p.x);
+        y0 = fminf(y0, p.y);
+        y1 = fmaxf(y1, p.y);
+        t = fmax(t, p.thr);
+    }
+    box[b] = BinInfo{x0, y0, x1, y1};
+    thr[b] = t;
+}
+
+// ======================================================== density raster ====
+//
+// One 256-thread workgroup computes a 64x64-cell block of a tile.  Lane l of
+// wave w owns column c0+l and rows r0+16w .. r0+16w+15 (16 accumulators).
+// The workgroup walks the candidate bins, classifies every point of those
+// bins against the block exactly (EMPTY: no cell inside the cut-off; FULL:
+// every cell inside; PARTIAL), queues the non-empty ones in bin order, and
+// consumes them in sub-batches of DT_SB points:
+//   1. a table Ey[p][row] = coef_p * exp2(dy2 * a_p) (and dy2 for PARTIAL
+//      points) is built cooperatively in LDS;
+//   2. each lane computes Ex = exp2(dx2 * a_p) once per point and adds
+//      Ex * Ey[p][row] to its 16 cells with one FMA each (FULL), or with the
+//      exact cut-off test fl64(dx2 + dy2) < thr first (PARTIAL);
+//   3. the FP32 partial sums are Kahan-folded after every sub-batch.
+// The order of points is fixed (bins in row-major order, points in bin order)
+// so results are deterministic.
+
+#define DT_SB 32
+#define DT_EYP 68   // floats per point row of the Ey table (64 + pad)
+#define DT_DYP 66   // doubles per point row of the dy2 table (64 + pad)
+#define DT_Q (DT_SB + 256)
+
+struct TileArgs {
+    GridDesc g;
+    const float *mask;   // tile values (nullable: all valid)
+    float *out;
+    PointIndex idx;
+};
+
+__global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
+{
+    __shared__ int s_bins[256];
+    __shared__ int s_wcnt[4];
+    __shared__ unsigned s_queue[DT_Q];
+    __shared__ float s_px[DT_SB];
+    __shared__ float s_py[DT_SB];
+    __shared__ float s_coef[DT_SB];
+    __shared__ double s_a[DT_SB];
+    __shared__ double s_thr[DT_SB];
+    __shared__ int s_part[DT_SB];
+    __shared__ __attribute__((aligned(16))) float s_ey[DT_SB * DT_EYP];
+    __shared__ __attribute__((aligned(16))) double s_dy2[DT_SB * DT_DYP];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const GridDesc g = A.g;
+    const double xll = g.xll, yll = g.yll, cs = g.cs, nrm05 = (double)g.nrows - 0.5;
+    const int64_t c0 = (int64_t)blockIdx.x * 64, r0 = (int64_t)blockIdx.y * 64;
+    const int64_t c1 = (c0 + 63 < g.ncols - 1) ? c0 + 63 : g.ncols - 1;
+    const int64_t r1 = (r0 + 63 < g.nrows - 1) ? r0 + 63 : g.nrows - 1;
+
+    // Block extent in cell-centre coordinates (exact N2 floats).
+    const float bx0 = cell_x(xll, cs, c0), bx1 = cell_x(xll, cs, c1);
+    const float by1 = cell_y(yll, cs, nrm05, r0), by0 = cell_y(yll, cs, nrm05, r1);
+    const float cx = cell_x(xll, cs, c0 + lane);
+
+    float acc[16], sum[16], comp[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) { acc[j] = 0.0f; sum[j] = 0.0f; comp[j] = 0.0f; }
+
+    const PointIndex &I = A.idx;
+    int bxa, bxb, bya, byb;
+    bin_range(I.bg, bx0, bx1, by0, by1, *I.reach_p, bxa, bxb, bya, byb);
+    const int nbx_r = bxb - bxa + 1;
+    const int nb_total = nbx_r * (byb - bya + 1);
+
+    int qn = 0;   // queue length (uniform, mirrored in registers)
+
+    auto process = [&](int cnt) {
+        // 1. point parameters
+        if (tid < cnt) {
+            unsigned q = s_queue[tid];
+            const PtRec p = I.pts[q & 0x7fffffffu];
+            s_px[tid] = p.x;
+            s_py[tid] = p.y;
+            s_coef[tid] = p.coef;
+            s_a[tid] = p.a;
+            s_thr[tid] = p.thr;
+            s_part[tid] = (int)(q >> 31);
+        }
+        __syncthreads();
+        // 2. Ey / dy2 tables: thread -> point (tid & 31), rows 8*(tid>>5) .. +7
+        {
+            const int p = tid & (DT_SB - 1), rg = tid >> 5;
+            if (p < cnt) {
+                const float py = s_py[p], coef = s_coef[p];
+                const double a = s_a[p];
+#pragma unroll
+                for (int k = 0; k < 8; k += 4) {
+                    float4 e4;
+                    double d2v[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int row = rg * 8 + k + u;
+                        float cy = cell_y(yll, cs, nrm05, r0 + row);
+                        double dy2 = sqd(cy - py);
+                        d2v[u] = dy2;
+                        float e = fast_exp2((float)(dy2 * a)) * coef;
+                        (&e4.x)[u] = e;
+                    }
+                    *(float4 *)&s_ey[p * DT_EYP + rg * 8 + k] = e4;
+                    *(double2 *)&s_dy2[p * DT_DYP + rg * 8 + k] = make_double2(d2v[0], d2v[1]);
+                    *(double2 *)&s_dy2[p * DT_DYP + rg * 8 + k + 2] = make_double2(d2v[2], d2v[3]);
+                }
+            }
+        }
+        __syncthreads();
+        // 3. accumulate
+        for (int p = 0; p < cnt; ++p) {
+            const float px = s_px[p];
+            const double a = s_a[p];
+            const float dx = cx - px;
+            const double dx2 = sqd(dx);
+            const float ex = fast_exp2((float)(dx2 * a));
+            const float4 *eyp = (const float4 *)&s_ey[p * DT_EYP + wave * 16];
+            float ey[16];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float4 v = eyp[k];
+                ey[4 * k] = v.x;
+                ey[4 * k + 1] = v.y;
+                ey[4 * k + 2] = v.z;
+                ey[4 * k + 3] = v.w;
+            }
+            const int part = __builtin_amdgcn_readfirstlane(s_part[p]);
+            if (!part) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) acc[j] = __builtin_fmaf(ex, ey[j], acc[j]);
+            } else {
+                const double thr = s_thr[p];
+                const double2 *dyp = (const double2 *)&s_dy2[p * DT_DYP + wave * 16];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    double2 v = dyp[k];
+                    bool in0 = __dadd_rn(dx2, v.x) < thr;
+                    bool in1 = __dadd_rn(dx2, v.y) < thr;
+                    acc[2 * k] = __builtin_fmaf(in0 ? ex : 0.0f, ey[2 * k], acc[2 * k]);
+                    acc[2 * k + 1] = __builtin_fmaf(in1 ? ex : 0.0f, ey[2 * k + 1], acc[2 * k + 1]);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            kahan_fold(acc[j], sum[j], comp[j]);
+            acc[j] = 0.0f;
+        }
+        __syncthreads();
+    };
+
+    for (int base = 0; base < nb_total; base += 256) {
+        // candidate bins of this chunk
+        const int k = base + tid;
+        bool cand = false;
+        int b = 0;
+        if (k < nb_total) {
+            b = (bya + k / nbx_r) * I.bg.nbx + (bxa + k % nbx_r);
+            if (I.bin_start[b + 1] > I.bin_start[b]) {
+                double g2 = box_gap2(I.bin_box[b], bx0, bx1, by0, by1);
+                cand = g2 < I.bin_thr[b];
+            }
+        }
+        int nb;
+        int slot = block_compact256(cand, s_wcnt, &nb);
+        if (cand) s_bins[slot] = b;
+        __syncthreads();
+        for (int kb = 0; kb < nb; ++kb) {
+            const int bb = s_bins[kb];
+            const int st = I.bin_start[bb], en = I.bin_start[bb + 1];
+            for (int off = st; off < en; off += 256) {
+                const int i = off + tid;
+                int cls = 0;   // 0 empty, 1 full, 2 partial
+                if (i < en) {
+                    const PtRec p = I.pts[i];
+                    if (p.thr > 0.0) {
+                        double near2 = sq2(gap1(p.x, bx0, bx1), gap1(p.y, by0, by1));
+                        if (near2 < p.thr) {
+                            double far2 = sq2(far1(p.x, bx0, bx1), far1(p.y, by0, by1));
+                            cls = far2 < p.thr ? 1 : 2;
+                        }
+                    }
+                }
+                int na;
+                int pos = block_compact256(cls != 0, s_wcnt, &na);
+                if (cls != 0) s_queue[qn + pos] = (unsigned)i | (cls == 2 ? 0x80000000u : 0u);
+                __syncthreads();
+                qn += na;
+                if (qn >= DT_SB) {
+                    int head = 0;
+                    while (qn - head >= DT_SB) {
+                        if (head > 0) {
+                            // move the next sub-batch to the queue front
+                            unsigned v = 0;
+                            if (tid < DT_SB) v = s_queue[head + tid];
+                            __syncthreads();
+                            if (tid < DT_SB) s_queue[tid] = v;
+                            __syncthreads();
+                        }
+                        process(DT_SB);
+                        head += DT_SB;
+                    }
+                    // keep the remainder at the front
+                    const int rem = qn - head;
+                    unsigned v = 0;
+                    if (tid < rem) v = s_queue[head + tid];
+                    __syncthreads();
+                    if (tid < rem) s_queue[tid] = v;
+                    __syncthreads();
+                    qn = rem;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (qn > 0) process(qn);
+
+    // write back: nodata cells keep the mask value
+    const int64_t col = c0 + lane;
+    if (col <= c1) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int64_t row = r0 + wave * 16 + j;
+            if (row <= r1) {
+                const int64_t ix = row * g.ncols + col;
+                float v = sum[j];
+                if (A.mask) {
+                    float m = A.mask[ix];
+                    if (m == g.nodata) v = m;
+                }
+                A.out[ix] = v;
+            }
+        }
+    }
+}
+
+// ==================================================== density at points ====
+//
+// One 256-thread workgroup owns 256 consecutive query points in bin order.
+// Source points of the candidate bins are staged through LDS 256 at a time;
+// every thread tests and accumulates all of them for its query.  Each source
+// uses its own bandwidth (sample-point estimator).  LOO excludes the query
+// itself (same index); duplicates at distance 0 are kept.
+
+struct PtsArgs {
+    PointIndex idx;
+    float *den;   // output, bin order
+    int loo;
+};
+
+__global__ __launch_bounds__(256) void k_point_density(PtsArgs A)
+{
+    __shared__ int s_bins[256];
+    __shared__ int s_wcnt[4];
+    __shared__ PtRec s_src[256];
+    __shared__ float s_red[4][4];
+
+    const PointIndex &I = A.idx;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t q = (int64_t)blockIdx.x * 256 + tid;
+    const bool qv = q < I.n;
+    float qx = 0.0f, qy = 0.0f;
+    if (qv) {
+        PtRec p = I.pts[q];
+        qx = p.x;
+        qy = p.y;
+    }
+    // bounding box of the group's queries
+    float mnx = qv ? qx : FLT_MAX, mxx = qv ? qx : -FLT_MAX, mny = qv ? qy : FLT_MAX, mxy = qv ? qy : -FLT_MAX;
+    for (int d = 32; d >= 1; d >>= 1) {
+        mnx = fminf(mnx, __shfl_xor(mnx, d, WAVE));
+        mxx = fmaxf(mxx, __shfl_xor(mxx, d, WAVE));
+        mny = fminf(mny, __shfl_xor(mny, d, WAVE));
+        mxy = fmaxf(mxy, __shfl_xor(mxy, d, WAVE));
+    }
+    if (lane == 0) {
+        s_red[wave][0] = mnx;
+        s_red[wave][1] = mxx;
+        s_red[wave][2] = mny;
+        s_red[wave][3] = mxy;
+    }
+    __syncthreads();
+    mnx = fminf(fminf(s_red[0][0], s_red[1][0]), fminf(s_red[2][0], s_red[3][0]));
+    mxx = fmaxf(fmaxf(s_red[0][1], s_red[1][1]), fmaxf(s_red[2][1], s_red[3][1]));
+    mny = fminf(fminf(s_red[0][2], s_red[1][2]), fminf(s_red[2][2], s_red[3][2]));
+    mxy = fmaxf(fmaxf(s_red[0][3], s_red[1][3]), fmaxf(s_red[2][3], s_red[3][3]));
+
+    int bxa, bxb, bya, byb;
+    bin_range(I.bg, mnx, mxx, mny, mxy, *I.reach_p, bxa, bxb, bya, byb);
+    const int nbx_r = bxb - bxa + 1;
+    const int nb_total = nbx_r * (byb - bya + 1);
+
+    float acc = 0.0f, sum = 0.0f, comp = 0.0f;
+    int nacc = 0;
+    for (int base = 0; base < nb_total; base += 256) {
+        const int k = base + tid;
+        bool cand = false;
+        int b = 0;
+        if (k < nb_total) {
+            b = (bya + k / nbx_r) * I.bg.nbx + (bxa + k % nbx_r);
+            if (I.bin_start[b + 1] > I.bin_start[b]) cand = box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < I.bin_thr[b];
+        }
+        int nb;
+        int slot = block_compact256(cand, s_wcnt, &nb);
+        if (cand) s_bins[slot] = b;
+        __syncthreads();
+        for (int kb = 0; kb < nb; ++kb) {
+            const int bb = s_bins[kb];
+            const int st = I.bin_start[bb], en = I.bin_start[bb + 1];
+            for (int off = st; off < en; off += 256) {
+                const int cnt = (en - off) < 256 ? (en - off) : 256;
+                if (tid < cnt) s_src[tid] = I.pts[off + tid];
+                __syncthreads();
+                for (int s = 0; s < cnt; ++s) {
+                    const PtRec p = s_src[s];
+                    const double d2 = sq2(p.x - qx, p.y - qy);
+                    bool in = d2 < p.thr;
+                    if (A.loo && (int64_t)(off + s) == q) in = false;
+                    const float e = fast_exp2((float)(d2 * p.a));
+                    acc = __builtin_fmaf(in ? e : 0.0f, p.coef, acc);
+                    if (++nacc == 32) {
+                        kahan_fold(acc, sum, comp);
+                        acc = 0.0f;
+                        nacc = 0;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    kahan_fold(acc, sum, comp);
+    if (qv) A.den[q] = sum;
+}
+
+// ============================================= batched LOO (fixed bandwidths) ==
+//
+// Up to LM_K candidate bandwidths h_k (ascending) in one pass over point
+// pairs.  For each pair the distance is computed once; candidates are visited
+// from the largest down and the loop stops for the whole wave as soon as no
+// lane is inside the cut-off of candidate k (the cut-off is monotone in h).
+// coef_k of a source depends on k through the candidate's edge-correction
+// weight: coefk[k*n + i] (bin order).
+
+#define LM_K MKDE_LM_K
+
+struct LooMultiArgs {
+    PointIndex idx;          // positions + bins (thr of the largest candidate)
+    const float *coefk;      // [K][n] bin order
+    const double *a_k;       // [K]
+    const double *thr_k;     // [K] ascending
+    int K;
+    float *den1;             // [K][n] bin order
+};
+
+__global__ __launch_bounds__(256) void k_loo_multi(LooMultiArgs A)
+{
+    __shared__ int s_bins[256];
+    __shared__ int s_wcnt[4];
+    __shared__ float s_sx[256];
+    __shared__ float s_sy[256];
+    __shared__ float s_coef[LM_K][256];
+    __shared__ double s_ak[LM_K];
+    __shared__ double s_tk[LM_K];
+    __shared__ float s_red[4][4];
+
+    const PointIndex &I = A.idx;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int K = A.K;
+    const int64_t n = I.n;
+    const int64_t q = (int64_t)blockIdx.x * 256 + tid;
+    const bool qv = q < n;
+    float qx = 0.0f, qy = 0.0f;
+    if (qv) {
+        PtRec p = I.pts[q];
+        qx = p.x;
+        qy = p.y;
+    }
+    if (tid < LM_K) {
+        s_ak[tid] = tid < K ? A.a_k[tid] : 0.0;
+        s_tk[tid] = tid < K ? A.thr_k[tid] : -1.0;
+    }
+    float mnx = qv ? qx : FLT_MAX, mxx = qv ? qx : -FLT_MAX, mny = qv ? qy : FLT_MAX, mxy = qv ? qy : -FLT_MAX;
+    for (int d = 32; d >= 1; d >>= 1) {
+        mnx = fminf(mnx, __shfl_xor(mnx, d, WAVE));
+        mxx = fmaxf(mxx, __shfl_xor(mxx, d, WAVE));
+        mny = fminf(mny, __shfl_xor(mny, d, WAVE));
+        mxy = fmaxf(mxy, __shfl_xor(mxy, d, WAVE));
+    }
+    if (lane == 0) {
+        s_red[wave][0] = mnx;
+        s_red[wave][1] = mxx;
+        s_red[wave][2] = mny;
+        s_red[wave][3] = mxy;
+    }
+    __syncthreads();
+    mnx = fminf(fminf(s_red[0][0], s_red[1][0]), fminf(s_red[2][0], s_red[3][0]));
+    mxx = fmaxf(fmaxf(s_red[0][1], s_red[1][1]), fmaxf(s_red[2][1], s_red[3][1]));
+    mny = fminf(fminf(s_red[0][2], s_red[1][2]), fminf(s_red[2][2], s_red[3][2]));
+    mxy = fmaxf(fmaxf(s_red[0][3], s_red[1][3]), fmaxf(s_red[2][3], s_red[3][3]));
+    const double tmax = s_tk[K - 1];
+
+    int bxa, bxb, bya, byb;
+    bin_range(I.bg, mnx, mxx, mny, mxy, sqrt(tmax) * 1.0000001 + 1e-30, bxa, bxb, bya, byb);
+    const int nbx_r = bxb - bxa + 1;
+    const int nb_total = nbx_r * (byb - bya + 1);
+
+    float acc[LM_K], sum[LM_K], comp[LM_K];
+#pragma unroll
+    for (int k = 0; k < LM_K; ++k) { acc[k] = 0.0f; sum[k] = 0.0f; comp[k] = 0.0f; }
+    int nacc = 0;
+
+    for (int base = 0; base < nb_total; base += 256) {
+        const int kk = base + tid;
+        bool cand = false;
+        int b = 0;
+        if (kk < nb_total) {
+            b = (bya + kk / nbx_r) * I.bg.nbx + (bxa + kk % nbx_r);
+            if (I.bin_start[b + 1] > I.bin_start[b]) cand = box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < tmax;
+        }
+        int nb;
+        int slot = block_compact256(cand, s_wcnt, &nb);
+        if (cand) s_bins[slot] = b;
+        __syncthreads();
+        for (int kb = 0; kb < nb; ++kb) {
+            const int bb = s_bins[kb];
+            const int st = I.bin_start[bb], en = I.bin_start[bb + 1];
+            for (int off = st; off < en; off += 256) {
+                const int cnt = (en - off) < 256 ? (en - off) : 256;
+                if (tid < cnt) {
+                    PtRec p = I.pts[off + tid];
+                    s_sx[tid] = p.x;
+                    s_sy[tid] = p.y;
+#pragma unroll
+                    for (int k = 0; k < LM_K; ++k) s_coef[k][tid] = k < K ? A.coefk[(int64_t)k * n + off + tid] : 0.0f;
+                }
+                __syncthreads();
+                for (int s = 0; s < cnt; ++s) {
+                    const double d2 = sq2(s_sx[s] - qx, s_sy[s] - qy);
+                    const bool self = (int64_t)(off + s) == q;
+#pragma unroll
+                    for (int k = LM_K - 1; k >= 0; --k) {
+                        if (k < K) {
+                            const bool in = (d2 < s_tk[k]) && !self;
+                            if (!__any(in)) break;
+                            const float e = fast_exp2((float)(d2 * s_ak[k]));
+                            acc[k] = __builtin_fmaf(in ? e : 0.0f, s_coef[k][s], acc[k]);
+                        }
+                    }
+                    if (++nacc == 32) {
+#pragma unroll
+                        for (int k = 0; k < LM_K; ++k) {
+                            kahan_fold(acc[k], sum[k], comp[k]);
+                            acc[k] = 0.0f;
+                        }
+                        nacc = 0;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < LM_K; ++k) {
+        kahan_fold(acc[k], sum[k], comp[k]);
+        if (qv && k < K) A.den1[(int64_t)k * n + q] = sum[k];
+    }
+}
+
+// ==================================================== range candidates ====
+// Candidate sets of N1 with the QUERY's own threshold (tests only).  Pass 1
+// counts (out_idx == nullptr), pass 2 fills at offsets (bin-order indexes).
+
+struct RangeArgs {
+    PointIndex idx;       // the query threshold is the point's own pts[q].thr
+    int64_t *counts;      // bin order
+    const int64_t *offsets;
+    int32_t *out_idx;
+};
+
+__global__ __launch_bounds__(256) void k_range(RangeArgs A)
+{
+    __shared__ int s_bins[256];
+    __shared__ int s_wcnt[4];
+    __shared__ float s_sx[256];
+    __shared__ float s_sy[256];
+    __shared__ float s_red[4][4];
+
+    const PointIndex &I = A.idx;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t q = (int64_t)blockIdx.x * 256 + tid;
+    const bool qv = q < I.n;
+    float qx = 0.0f, qy = 0.0f;
+    double qt = -1.0;
+    if (qv) {
+        PtRec p = I.pts[q];
+        qx = p.x;
+        qy = p.y;
+        qt = p.thr;
+    }
+    float mnx = qv ? qx : FLT_MAX, mxx = qv ? qx : -FLT_MAX, mny = qv ? qy : FLT_MAX, mxy = qv ? qy : -FLT_MAX;
+    for (int d = 32; d >= 1; d >>= 1) {
+        mnx = fminf(mnx, __shfl_xor(mnx, d, WAVE));
+        mxx = fmaxf(mxx, __shfl_xor(mxx, d, WAVE));
+        mny = fminf(mny, __shfl_xor(mny, d, WAVE));
+        mxy = fmaxf(mxy, __shfl_xor(mxy, d, WAVE));
+    }
+    if (lane == 0) {
+        s_red[wave][0] = mnx;
+        s_red[wave][1] = mxx;
+        s_red[wave][2] = mny;
+        s_red[wave][3] = mxy;
+    }
+    __syncthreads();
+    mnx = fminf(fminf(s_red[0][0], s_red[1][0]), fminf(s_red[2][0], s_red[3][0]));
+    mxx = fmaxf(fmaxf(s_red[0][1], s_red[1][1]), fmaxf(s_red[2][1], s_red[3][1]));
+    mny = fminf(fminf(s_red[0][2], s_red[1][2]), fminf(s_red[2][2], s_red[3][2]));
+    mxy = fmaxf(fmaxf(s_red[0][3], s_red[1][3]), fmaxf(s_red[2][3], s_red[3][3]));
+
+    int bxa, bxb, bya, byb;
+    const double reach = *I.reach_p;
+    const double qthr_max = reach * reach;
+    bin_range(I.bg, mnx, mxx, mny, mxy, reach, bxa, bxb, bya, byb);
+    const int nbx_r = bxb - bxa + 1;
+    const int nb_total = nbx_r * (byb - bya + 1);
+    int64_t cnt_q = 0;
+    int64_t wpos = (A.out_idx && qv) ? A.offsets[q] : 0;
+    for (int base = 0; base < nb_total; base += 256) {
+        const int kk = base + tid;
+        bool cand = false;
+        int b = 0;
+        if (kk < nb_total) {
+            b = (bya + kk / nbx_r) * I.bg.nbx + (bxa + kk % nbx_r);
+            if (I.bin_start[b + 1] > I.bin_start[b])
+                cand = box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < qthr_max;
+        }
+        int nb;
+        int slot = block_compact256(cand, s_wcnt, &nb);
+        if (cand) s_bins[slot] = b;
+        __syncthreads();
+        for (int kb = 0; kb < nb; ++kb) {
+            const int bb = s_bins[kb];
+            const int st = I.bin_start[bb], en = I.bin_start[bb + 1];
+            for (int off = st; off < en; off += 256) {
+                const int cnt = (en - off) < 256 ? (en - off) : 256;
+                if (tid < cnt) {
+                    PtRec p = I.pts[off + tid];
+                    s_sx[tid] = p.x;
+                    s_sy[tid] = p.y;
+                }
+                __syncthreads();
+                if (qv) {
+                    for (int s = 0; s < cnt; ++s) {
+                        const double d2 = sq2(qx - s_sx[s], qy - s_sy[s]);
+                        if (d2 < qt) {
+                            if (A.out_idx) A.out_idx[wpos++] = off + s;
+                            cnt_q++;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    if (qv && !A.out_idx) A.counts[q] = cnt_q;
+}
+
+// ============================================================ boundary ====
+
+__global__ void k_valid(int64_t ncells, const float *mask, float nodata, uint8_t *valid)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ncells) valid[i] = mask[i] != nodata ? 1 : 0;
+}
+
+// MarkBoundary: a valid cell is a boundary cell iff it is on the outer rows /
+// columns or one of its 8 neighbours is nodata (tiled:3993-4059).  Boundary
+// cell centres are appended to blist (order irrelevant: only minima are taken).
+__global__ void k_boundary(GridDesc g, const uint8_t *valid, uint8_t *flags, float2 *blist,
+                           unsigned long long *bcount)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nc = g.ncols, nr = g.nrows;
+    if (i >= nc * nr) return;
+    const int64_t r = i / nc, c = i - r * nc;
+    uint8_t f = 0;
+    if (valid[i]) {
+        if (r == 0 || r == nr - 1 || c == 0 || c == nc - 1) {
+            f = 1;
+        } else {
+            const uint8_t *up = valid + (r - 1) * nc + c, *mid = valid + r * nc + c, *dn = valid + (r + 1) * nc + c;
+            f = !(up[-1] & up[0] & up[1] & mid[-1] & mid[1] & dn[-1] & dn[0] & dn[1]);
+        }
+    }
+    flags[i] = f;
+    if (f) {
+        unsigned long long k = atomicAdd(bcount, 1ull);
+        const double nrm05 = (double)nr - 0.5;
+        blist[k] = make_float2(cell_x(g.xll, g.cs, c), cell_y(g.yll, g.cs, nrm05, r));
+    }
+}
+
+// CalcDist2Boundary (CPU semantics, tiled:4091-4131): min over boundary cells
+// of fl64(dx^2 + dy^2), DBL_MAX if none.  Boundary centres streamed via LDS.
+__global__ __launch_bounds__(256) void k_dist_boundary(int64_t n, const float *x, const float *y, const float2 *blist,
+                                                       int64_t nb, double *dist)
+{
+    __shared__ float2 s_b[1024];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const float px = i < n ? x[i] : 0.0f, py = i < n ? y[i] : 0.0f;
+    double m = DBL_MAX;
+    for (int64_t base = 0; base < nb; base += 1024) {
+        const int cnt = (nb - base) < 1024 ? (int)(nb - base) : 1024;
+        for (int t = threadIdx.x; t < cnt; t += blockDim.x) s_b[t] = blist[base + t];
+        __syncthreads();
+        for (int t = 0; t < cnt; ++t) {
+            const float2 b = s_b[t];
+            const double d2 = sq2(b.x - px, b.y - py);
+            m = d2 < m ? d2 : m;
+        }
+        __syncthreads();
+    }
+    if (i < n) dist[i] = m;
+}
+
+// ===================================================== edge correction ====
+//
+// EdgeCorrectionWeightsExact with O(R) work per point instead of O(R^2).
+// The integrand is separable on the mask grid:
+//   K(c, r) = exp2(dx2_c a) * exp2(dy2_r a) / (2 pi h2f)
+// and for a fixed row the cells inside the cut-off form one interval of
+// columns (the reference's test is monotone in |dx|).  One wave handles one
+// point: columns of the NARROW box (tiled:2716-2727) are processed in chunks
+// of EC_CW, with a float-float prefix sum of exp2(dx2 a) in LDS; lanes then
+// take rows, find each row's exact column interval, intersect it with the
+// row's valid-cell runs and add exp2(dy2 a) * (prefix differences).
+// Output ew = 1/(cellArea * sum / (2 pi h2f)); points whose boundary distance
+// is >= 9 h2f were given weight 1 by k_ec_active.
+
+#define EC_CW 2048
+
+struct EcArgs {
+    GridDesc g;
+    MaskRuns runs;
+    const float *x, *y;     // original order
+    const float *h;         // original order (nullable when h_fixed > 0)
+    float h_fixed;
+    const int *list;        // active points
+    float *ew;              // original order
+    unsigned long long *n_eval;
+};
+
+__global__ void k_ec_active(int64_t n, const double *dist, const float *h, float h_fixed, float *ew, int *list,
+                            int *count)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float hh = h_fixed > 0.0f ? h_fixed : h[i];
+    const double h2 = (double)(float)(hh * hh);
+    if (dist[i] >= 9.0 * h2) {
+        ew[i] = 1.0f;
+    } else {
+        int k = atomicAdd(count, 1);
+        list[k] = (int)i;
+    }
+}
+
+// NARROW box with the reference's macro semantics (Utilities.h:16-17,30-31;
+// tiled:2716-2727): size_t arithmetic for row_upper / col_upper.
+__device__ __forceinline__ void narrow_box(const GridDesc &g, float px, float py, float h, int &rl, int &ru, int &cl,
+                                           int &cu)
+{
+    const uint64_t nR = (uint64_t)g.nrows, nC = (uint64_t)g.ncols;
+    const double yll = g.yll, xll = g.xll, cs = g.cs, hh = h;
+    const int k_top = (int)(((double)py + 3.0 * hh - yll) / cs);
+    const int r = (int)(int32_t)(uint32_t)(nR - 1 - (uint64_t)(int64_t)k_top);
+    rl = 0 < r ? r : 0;
+    const int k_bot = (int)(((double)py - 3.0 * hh - yll) / cs);
+    const uint64_t yu = nR - 1 - (uint64_t)(int64_t)k_bot;
+    ru = (int)(int32_t)(uint32_t)((nR - 1 < yu) ? nR - 1 : yu);
+    const int k_left = (int)(((double)px - 3.0 * hh - xll) / cs);
+    cl = 0 < k_left ? k_left : 0;
+    const int k_right = (int)(((double)px + 3.0 * hh - xll) / cs);
+    const uint64_t kr = (uint64_t)(int64_t)k_right;
+    cu = (int)(int32_t)(uint32_t)((nC - 1 < kr) ? nC - 1 : kr);
+}
+
+__device__ __forceinline__ bool pred_col(double xll, double cs, int64_t c, float px, double dy2, double thr)
+{
+    const float dx = cell_x(xll, cs, c) - px;
+    return __dadd_rn(sqd(dx), dy2) < thr;
+}
+
+// Exact column interval [lo, hi] within [cl, cu] of the cells of one row that
+// are inside the cut-off; lo > hi when empty.  cn is the column of [cl, cu]
+// nearest to px (|dx| is smallest there).
+__device__ __forceinline__ void row_interval(double xll, double cs, int cl, int cu, int cn, float px, double dy2,
+                                             double thr, int &lo, int &hi)
+{
+    if (!pred_col(xll, cs, cn, px, dy2, thr)) {
+        lo = 1;
+        hi = 0;
+        return;
+    }
+    const double rem = thr - dy2;
+    const double w = rem > 0.0 ? sqrt(rem) : 0.0;
+    double fh = floor(((double)px + w - xll) / cs - 0.5);
+    double fl = ceil(((double)px - w - xll) / cs - 0.5);
+    int h_e = (int)fmin(fmax(fh, (double)cn), (double)cu);
+    int l_e = (int)fmin(fmax(fl, (double)cl), (double)cn);
+    while (h_e < cu && pred_col(xll, cs, h_e + 1, px, dy2, thr)) ++h_e;
+    while (h_e > cn && !pred_col(xll, cs, h_e, px, dy2, thr)) --h_e;
+    while (l_e > cl && pred_col(xll, cs, l_e - 1, px, dy2, thr)) --l_e;
+    while (l_e < cn && !pred_col(xll, cs, l_e, px, dy2, thr)) ++l_e;
+    lo = l_e;
+    hi = h_e;
+}
+
+// First run index in [rs, re) whose end >= c (binary search).
+__device__ __forceinline__ int64_t first_run_ending_at_or_after(const int2 *runs, int64_t rs, int64_t re, int c)
+{
+    int64_t lo = rs, hi = re;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (runs[mid].y < c) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(64) void k_ec(EcArgs A)
+{
+    __shared__ float s_ph[EC_CW];
+    __shared__ float s_pl[EC_CW];
+
+    const int lane = threadIdx.x;
+    const int p = A.list[blockIdx.x];
+    const GridDesc g = A.g;
+    const double xll = g.xll, yll = g.yll, cs = g.cs, nrm05 = (double)g.nrows - 0.5;
+    const float px = A.x[p], py = A.y[p];
+    const float hh = A.h_fixed > 0.0f ? A.h_fixed : A.h[p];
+    const double h2 = (double)(float)(hh * hh);
+    const double thr = 9.0 * h2;
+    const double a = -MKDE_LOG2E / (2.0 * h2);
+
+    int rl, ru, cl, cu;
+    narrow_box(g, px, py, hh, rl, ru, cl, cu);
+    ff acc = {0.0f, 0.0f};
+    unsigned long long cnt = 0;
+    if (rl <= ru && cl <= cu) {
+        const int cn = clampi((int64_t)floor(((double)px - xll) / cs), cl, cu);
+        for (int ca = cl; ca <= cu; ca += EC_CW) {
+            const int cb = (ca + EC_CW - 1 < cu) ? ca + EC_CW - 1 : cu;
+            const int w = cb - ca + 1;
+            // lane owns columns [ca + lane*per, ...)
+            const int per = (w + WAVE - 1) / WAVE;
+            const int my0 = lane * per;
+            ff run = {0.0f, 0.0f};
+            for (int t = 0; t < per; ++t) {
+                const int k = my0 + t;
+                if (k < w) {
+                    const float dx = cell_x(xll, cs, ca + k) - px;
+                    const float ex = fast_exp2((float)(sqd(dx) * a));
+                    run = ff_add_f(run, ex);
+                    s_ph[k] = run.hi;
+                    s_pl[k] = run.lo;
+                }
+            }
+            // exclusive wave scan of the lane totals
+            ff tot = run;
+            ff inc = tot;
+            for (int d = 1; d < WAVE; d <<= 1) {
+                ff o = ff_shfl_up(inc, d);
+                if (lane >= d) inc = ff_add(inc, o);
+            }
+            ff offv = ff_add(inc, ff_neg(tot));
+            for (int t = 0; t < per; ++t) {
+                const int k = my0 + t;
+                if (k < w) {
+                    ff v = ff_add(ff{s_ph[k], s_pl[k]}, offv);
+                    s_ph[k] = v.hi;
+                    s_pl[k] = v.lo;
+                }
+            }
+            __syncthreads();
+            // rows
+            for (int r = rl + lane; r <= ru; r += WAVE) {
+                const float cy = cell_y(yll, cs, nrm05, r);
+                const double dy2 = sqd(cy - py);
+                if (!(dy2 < thr)) continue;
+                int lo, hi;
+                row_interval(xll, cs, cl, cu, cn, px, dy2, thr, lo, hi);
+                if (lo < ca) lo = ca;
+                if (hi > cb) hi = cb;
+                if (lo > hi) continue;
+                const int64_t rs = A.runs.run_ptr[r], re = A.runs.run_ptr[r + 1];
+                ff seg = {0.0f, 0.0f};
+                for (int64_t k = first_run_ending_at_or_after(A.runs.run_se, rs, re, lo); k < re; ++k) {
+                    const int2 rr = A.runs.run_se[k];
+                    if (rr.x > hi) break;
+                    const int s = rr.x > lo ? rr.x : lo;
+                    const int e = rr.y < hi ? rr.y : hi;
+                    ff ps = ff{s_ph[e - ca], s_pl[e - ca]};
+                    if (s > ca) ps = ff_add(ps, ff_neg(ff{s_ph[s - 1 - ca], s_pl[s - 1 - ca]}));
+                    seg = ff_add(seg, ps);
+                    cnt += (unsigned long long)(e - s + 1);
+                }
+                const float ey = fast_exp2((float)(dy2 * a));
+                acc = ff_add(acc, ff_mul_f(seg, ey));
+            }
+            __syncthreads();
+        }
+    }
+    // wave reduction
+    for (int d = 32; d >= 1; d >>= 1) {
+        acc = ff_add(acc, ff_shfl_xor(acc, d));
+        cnt += __shfl_xor(cnt, d, WAVE);
+    }
+    if (lane == 0) {
+        const double ec = ((double)acc.hi + (double)acc.lo) * (double)(float)(g.cs * g.cs) /
+                          (h2 * (double)MKDE_TWO_PI_F);
+        A.ew[p] = (float)(1.0 / ec);
+        if (A.n_eval) atomicAdd(A.n_eval, cnt);
+    }
+}
+
+// Exact count of (point, valid cell) pairs inside the cut-off on a tile (the
+// density kernel's algorithmic work), one wave per point, rows on lanes.
+struct CountArgs {
+    GridDesc g;
+    MaskRuns runs;      // valid runs of the tile (all-valid tiles have one run per row)
+    PointIndex idx;
+    unsigned long long *total;
+};
+
+__global__ __launch_bounds__(64) void k_count_pairs(CountArgs A)
+{
+    const int lane = threadIdx.x;
+    const int64_t i = blockIdx.x;
+    const PtRec P = A.idx.pts[i];
+    const GridDesc g = A.g;
+    unsigned long long cnt = 0;
+    if (P.thr > 0.0) {
+        const double xll = g.xll, yll = g.yll, cs = g.cs, nrm05 = (double)g.nrows - 0.5;
+        const double h = sqrt(P.thr / 9.0);
+        int rl, ru, cl, cu;
+        narrow_box(g, P.x, P.y, (float)(h * 1.000001) + 1e-6f, rl, ru, cl, cu);
+        if (rl <= ru && cl <= cu) {
+            const int cn = clampi((int64_t)floor(((double)P.x - xll) / cs), cl, cu);
+            for (int r = rl + lane; r <= ru; r += WAVE) {
+                const float cy = cell_y(yll, cs, nrm05, r);
+                const double dy2 = sqd(cy - P.y);
+                if (!(dy2 < P.thr)) continue;
+                int lo, hi;
+                row_interval(xll, cs, cl, cu, cn, P.x, dy2, P.thr, lo, hi);
+                if (lo > hi) continue;
+                const int64_t rs = A.runs.run_ptr[r], re = A.runs.run_ptr[r + 1];
+                for (int64_t k = first_run_ending_at_or_after(A.runs.run_se, rs, re, lo); k < re; ++k) {
+                    const int2 rr = A.runs.run_se[k];
+                    if (rr.x > hi) break;
+                    const int s = rr.x > lo ? rr.x : lo;
+                    const int e = rr.y < hi ? rr.y : hi;
+                    cnt += (unsigned long long)(e - s + 1);
+                }
+            }
+        }
+    }
+    for (int d = 32; d >= 1; d >>= 1) cnt += __shfl_xor(cnt, d, WAVE);
+    if (lane == 0 && cnt) atomicAdd(A.total, cnt);
+}
+
+// reach = sqrt(max bin threshold) (world units), slightly enlarged so that the
+// bin range computed from it is a superset.
+__global__ __launch_bounds__(256) void k_reach(int nbins, const double *bin_thr, double *reach)
+{
+    __shared__ double s[4];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nbins; b += 256) m = fmax(m, bin_thr[b]);
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, WAVE));
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmax(fmax(s[0], s[1]), fmax(s[2], s[3]));
+        reach[0] = m > 0.0 ? sqrt(m) * 1.000001 + 1e-6 : 0.0;
+    }
+}
+
+// g = exp(S / n): the geometric mean of the pilot densities (compGML, tiled:3807-3814).
+__global__ void k_gml(const double *S, int64_t n, double *g)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) g[0] = exp(S[0] / (double)n);
+}
+
+// ============================================================ reductions ====
+
+// partial[k * nblk + b] = sum over the block's points of log(max(den, 1e-14f))
+// (or of log(den) when clamp == 0), in FP64.  Sequential second pass.
+__global__ __launch_bounds__(256) void k_sumlog_partial(int64_t n, const float *den, int clamp, double *partial)
+{
+    __shared__ double s[4];
+    const int k = blockIdx.y;
+    const float *d = den + (int64_t)k * n;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        float v = d[i];
+        if (clamp) v = v < MKDE_EPS_DEN ? MKDE_EPS_DEN : v;
+        acc += log((double)v);
+    }
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, WAVE);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(int64_t)k * gridDim.x + blockIdx.x] = ((s[0] + s[1]) + s[2]) + s[3];
+}
+
+__global__ void k_sum_final(int nblk, int K, const double *partial, double *out)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)k * nblk + b];
+    out[k] = s;
+}
+
+// Abramson update h_i = h0 * powf(den0_i / g, alpha) (tiled:571, kde_kernel_kdtr.cu:1007-1037),
+// den0 in bin order, h written in original order.
+__global__ void k_abramson(int64_t n, const int *perm, const float *den0_sorted, const double *g, float h0,
+                           float alpha, float *h_orig)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float ratio = (float)((double)den0_sorted[i] / g[0]);
+    h_orig[perm[i]] = h0 * powf(ratio, alpha);
+}
+
+// scatter a bin-order array back to original order
+__global__ void k_unsort_f32(int64_t n, const int *perm, const float *src, float *dst)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[perm[i]] = src[i];
+}
+
+// gather an original-order array into bin order
+__global__ void k_sort_f32(int64_t n, const int *perm, const float *src, float *dst)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[perm[i]];
+}
+
+// coefk[k*n + i] = w * ew_k / (2 pi h2_k), bin order
+__global__ void k_coef_k(int64_t n, const int *perm, const float *w_orig, const float *ew_k_orig, double h2k,
+                         float *coef_out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = perm[i];
+    coef_out[i] = (float)((double)w_orig[o] * (double)ew_k_orig[o] / (h2k * (double)MKDE_TWO_PI_F));
+}
+
+__global__ void k_fill_f32(int64_t n, float v, float *dst)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+// ====================================================== host launchers ====
+
+extern "C++" {
+namespace mkde_dev {
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t prep_points(hipStream_t s, int64_t n, const int *perm, const float *xs, const float *ys,
+                       const float *w, const float *h, const float *ew, float h_fixed, PtRec *out)
+{
+    if (n > 0) hipLaunchKernelGGL(k_prep_points, dim3(nblk(n, 256)), dim3(256), 0, s, n, perm, xs, ys, w, h, ew,
+                                  h_fixed, out);
+    return hipGetLastError();
+}
+
+hipError_t bin_stats(hipStream_t s, int nbins, const int *bin_start, const PtRec *pts, BinInfo *box, double *thr)
+{
+    hipLaunchKernelGGL(k_bin_stats, dim3(nblk(nbins, 256)), dim3(256), 0, s, nbins, bin_start, pts, box, thr);
+    return hipGetLastError();
+}
+
+hipError_t density_tile(hipStream_t s, const GridDesc &g, const float *mask, float *out, const PointIndex &idx)
+{
+    TileArgs A{g, mask, out, idx};
+    dim3 grid((unsigned)((g.ncols + 63) / 64), (unsigned)((g.nrows + 63) / 64));
+    hipLaunchKernelGGL(k_density_tile, grid, dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t point_density(hipStream_t s, const PointIndex &idx, int loo, float *den_sorted)
+{
+    PtsArgs A{idx, den_sorted, loo};
+    if (idx.n > 0) hipLaunchKernelGGL(k_point_density, dim3(nblk(idx.n, 256)), dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t loo_multi(hipStream_t s, const PointIndex &idx, const float *coefk, const double *a_k,
+                     const double *thr_k, int K, float *den1)
+{
+    LooMultiArgs A{idx, coefk, a_k, thr_k, K, den1};
+    if (idx.n > 0) hipLaunchKernelGGL(k_loo_multi, dim3(nblk(idx.n, 256)), dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t range_count(hipStream_t s, const PointIndex &idx, int64_t *counts)
+{
+    RangeArgs A{idx, counts, nullptr, nullptr};
+    if (idx.n > 0) hipLaunchKernelGGL(k_range, dim3(nblk(idx.n, 256)), dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t range_fill(hipStream_t s, const PointIndex &idx, const int64_t *offsets, int32_t *out_idx)
+{
+    RangeArgs A{idx, nullptr, offsets, out_idx};
+    if (idx.n > 0) hipLaunchKernelGGL(k_range, dim3(nblk(idx.n, 256)), dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t valid_cells(hipStream_t s, int64_t ncells, const float *mask, float nodata, uint8_t *valid)
+{
+    hipLaunchKernelGGL(k_valid, dim3(nblk(ncells, 256)), dim3(256), 0, s, ncells, mask, nodata, valid);
+    return hipGetLastError();
+}
+
+hipError_t boundary(hipStream_t s, const GridDesc &g, const uint8_t *valid, uint8_t *flags, float2 *blist,
+                    unsigned long long *bcount)
+{
+    int64_t nc = g.ncols * g.nrows;
+    hipLaunchKernelGGL(k_boundary, dim3(nblk(nc, 256)), dim3(256), 0, s, g, valid, flags, blist, bcount);
+    return hipGetLastError();
+}
+
+hipError_t dist_boundary(hipStream_t s, int64_t n, const float *x, const float *y, const float2 *blist, int64_t nb,
+                         double *dist)
+{
+    if (n > 0) hipLaunchKernelGGL(k_dist_boundary, dim3(nblk(n, 256)), dim3(256), 0, s, n, x, y, blist, nb, dist);
+    return hipGetLastError();
+}
+
+hipError_t ec_active(hipStream_t s, int64_t n, const double *dist, const float *h, float h_fixed, float *ew,
+                     int *list, int *count)
+{
+    if (n > 0) hipLaunchKernelGGL(k_ec_active, dim3(nblk(n, 256)), dim3(256), 0, s, n, dist, h, h_fixed, ew, list,
+                                  count);
+    return hipGetLastError();
+}
+
+hipError_t ec(hipStream_t s, int nactive, const GridDesc &g, const MaskRuns &runs, const float *x, const float *y,
+              const float *h, float h_fixed, const int *list, float *ew, unsigned long long *n_eval)
+{
+    EcArgs A{g, runs, x, y, h, h_fixed, list, ew, n_eval};
+    if (nactive > 0) hipLaunchKernelGGL(k_ec, dim3((unsigned)nactive), dim3(64), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t count_pairs(hipStream_t s, const GridDesc &g, const MaskRuns &runs, const PointIndex &idx,
+                       unsigned long long *total)
+{
+    CountArgs A{g, runs, idx, total};
+    if (idx.n > 0) hipLaunchKernelGGL(k_count_pairs, dim3((unsigned)idx.n), dim3(64), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t reach(hipStream_t s, int nbins, const double *bin_thr, double *reach_out)
+{
+    hipLaunchKernelGGL(k_reach, dim3(1), dim3(256), 0, s, nbins, bin_thr, reach_out);
+    return hipGetLastError();
+}
+
+hipError_t gml(hipStream_t s, const double *S, int64_t n, double *g)
+{
+    hipLaunchKernelGGL(k_gml, dim3(1), dim3(64), 0, s, S, n, g);
+    return hipGetLastError();
+}
+
+hipError_t sumlog(hipStream_t s, int64_t n, int K, const float *den, int clamp, double *partial, int nb, double *out)
+{
+    hipLaunchKernelGGL(k_sumlog_partial, dim3((unsigned)nb, (unsigned)K), dim3(256), 0, s, n, den, clamp, partial);
+    hipLaunchKernelGGL(k_sum_final, dim3(nblk(K, 64)), dim3(64), 0, s, nb, K, partial, out);
+    return hipGetLastError();
+}
+
+hipError_t abramson(hipStream_t s, int64_t n, const int *perm, const float *den0_sorted, const double *g, float h0,
+                    float alpha, float *h_orig)
+{
+    if (n > 0) hipLaunchKernelGGL(k_abramson, dim3(nblk(n, 256)), dim3(256), 0, s, n, perm, den0_sorted, g, h0,
+                                  alpha, h_orig);
+    return hipGetLastError();
+}
+
+hipError_t unsort_f32(hipStream_t s, int64_t n, const int *perm, const float *src, float *dst)
+{
+    if (n > 0) hipLaunchKernelGGL(k_unsort_f32, dim3(nblk(n, 256)), dim3(256), 0, s, n, perm, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t sort_f32(hipStream_t s, int64_t n, const int *perm, const float *src, float *dst)
+{
+    if (n > 0) hipLaunchKernelGGL(k_sort_f32, dim3(nblk(n, 256)), dim3(256), 0, s, n, perm, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t coef_k(hipStream_t s, int64_t n, const int *perm, const float *w, const float *ew_k, double h2k,
+                  float *coef_out)
+{
+    if (n > 0) hipLaunchKernelGGL(k_coef_k, dim3(nblk(n, 256)), dim3(256), 0, s, n, perm, w, ew_k, h2k, coef_out);
+    return hipGetLastError();
+}
+
+hipError_t fill_f32(hipStream_t s, int64_t n, float v, float *dst)
+{
+    if (n > 0) hipLaunchKernelGGL(k_fill_f32, dim3(nblk(n, 256)), dim3(256), 0, s, n, v, dst);
+    return hipGetLastError();
+}
+
+}  // namespace mkde_dev
+}

@@ -1,0 +1,234 @@
+"""GPU parity: every kernel of the hot path against the oracle (the C
+restatement of the reference CPU path) on the same seeded inputs.
+
+Bars: bit-exact for flags, boundary distances, candidate sets and pair counts;
+1e-5 relative against the FP64 oracle for densities, EC weights, bandwidths.
+"""
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5  # relative tolerance of densities / weights against the FP64 oracle
+
+
+def rel_err(got, ref):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    den = np.where(ref != 0, np.abs(ref), 1.0)
+    return np.abs(got - ref) / den
+
+
+def assert_density_close(got, ref64, tol=REL):
+    zero = ref64 == 0
+    assert np.all(got[zero] == 0), "cells outside every cut-off must be exactly 0"
+    e = rel_err(got[~zero], ref64[~zero])
+    assert e.max(initial=0.0) <= tol, "max rel err %.3g at %s" % (e.max(), np.argmax(e))
+
+
+@pytest.fixture(scope="module")
+def scene(orc):
+    """Clustered weighted points over a 300x300 mask (disk with holes)."""
+    x, y, w = cases.thomas_points(12, 250, 12.0, 300.0, seed=20260101)
+    x, y, w = x[:3000], y[:3000], w[:3000]
+    g = orc.grid(300, 300, 0.0, 0.0, 1.0)
+    mask = cases.disk_mask(300, 300, 140, n_holes=4, hole_min=10, hole_max=40, seed=20260101)
+    return dict(x=x, y=y, w=w, g=g, mask=mask)
+
+
+def mgrid(M, g):
+    return M.grid(g.ncols, g.nrows, g.xll, g.yll, g.cs, g.nodata)
+
+
+def test_boundary_flags_and_distances(ctx, orc, scene):
+    import mgpukde_tiled_amd as M
+
+    s = scene
+    ctx.points_upload(s["x"], s["y"], s["w"])
+    ctx.mask_upload(mgrid(M, s["g"]), s["mask"])
+    flags, nb = ctx.MarkBoundary()
+    oflags, onb = orc.mark_boundary(s["g"], s["mask"])
+    assert nb == onb
+    assert np.array_equal(flags, oflags)
+    d = ctx.CalcDist2Boundary()
+    od = orc.dist_to_boundary(s["x"], s["y"], s["g"], oflags)
+    assert np.array_equal(d, od), "boundary distances must be bit-exact"
+
+
+@pytest.mark.parametrize("h", [6.0, 17.5])
+def test_ec_weights_fixed(ctx, orc, scene, h):
+    import mgpukde_tiled_amd as M
+
+    s = scene
+    ctx.points_upload(s["x"], s["y"], s["w"])
+    ctx.mask_upload(mgrid(M, s["g"]), s["mask"])
+    ctx.set_bandwidth(h)
+    ew, ne = ctx.EdgeCorrectionWeightsExact()
+    oflags, _ = orc.mark_boundary(s["g"], s["mask"])
+    od = orc.dist_to_boundary(s["x"], s["y"], s["g"], oflags)
+    hs = np.full(len(s["x"]), h, np.float32)
+    _, ew64, one = orc.ec_weights(s["x"], s["y"], od, hs, s["g"], s["mask"])
+    assert ne == one, "EC pair count must equal the oracle's"
+    skipped = ew64 == 1.0
+    assert np.all(ew[skipped] == 1.0)
+    assert rel_err(ew, ew64).max() <= REL
+
+
+def test_ec_weights_adaptive(ctx, orc, scene):
+    import mgpukde_tiled_amd as M
+
+    s = scene
+    r = np.random.default_rng(5)
+    hs = r.uniform(3.0, 25.0, len(s["x"])).astype(np.float32)
+    ctx.points_upload(s["x"], s["y"], s["w"])
+    ctx.mask_upload(mgrid(M, s["g"]), s["mask"])
+    ctx.set_bandwidths(hs)
+    ew, ne = ctx.EdgeCorrectionWeightsExact()
+    oflags, _ = orc.mark_boundary(s["g"], s["mask"])
+    od = orc.dist_to_boundary(s["x"], s["y"], s["g"], oflags)
+    _, ew64, one = orc.ec_weights(s["x"], s["y"], od, hs, s["g"], s["mask"])
+    assert ne == one
+    assert rel_err(ew, ew64).max() <= REL
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_density_tile(ctx, orc, scene, adaptive):
+    import mgpukde_tiled_amd as M
+
+    s = scene
+    n = len(s["x"])
+    r = np.random.default_rng(7)
+    hs = r.uniform(4.0, 30.0, n).astype(np.float32) if adaptive else np.full(n, 12.0, np.float32)
+    ew = r.uniform(0.9, 1.6, n).astype(np.float32)
+    ctx.points_upload(s["x"], s["y"], s["w"])
+    ctx.set_bandwidths(hs)
+    ctx.set_ec_weights(ew)
+    # a tile with its own origin, not aligned to 64-cell blocks
+    tile = orc.grid(230, 170, 20.0, 60.0, 1.0)
+    tmask = s["mask"][70:240, 20:250]
+    got = ctx.KernelDensityEstimation_tile(mgrid(M, tile), tmask)
+    o32, o64, ne = orc.density_tile(s["x"], s["y"], s["w"], ew, hs, tile, tmask)
+    nod = tmask == cases.NODATA
+    assert np.all(got[nod] == cases.NODATA)
+    assert_density_close(got[~nod], o64[~nod])
+    assert ctx.count_density_pairs(mgrid(M, tile), tmask) == ne
+
+
+def test_density_tile_fine_grid_mode0(ctx, orc):
+    """Mode-0 style: 100 uniform points, 1000x1000 raster at cs 0.1."""
+    import mgpukde_tiled_amd as M
+
+    x, y = cases.uniform_points(100, seed=20260001)
+    w = np.ones(100, np.float32)
+    h = cases.plain_small_bandwidth(x, y)
+    ncols, nrows, xll, yll, cs = cases.mode0_density_tile(0.1)
+    tile = orc.grid(ncols, nrows, xll, yll, cs)
+    ew = np.random.default_rng(3).uniform(1.0, 2.0, 100).astype(np.float32)
+    ctx.points_upload(x, y, w)
+    ctx.set_bandwidth(h)
+    ctx.set_ec_weights(ew)
+    got = ctx.KernelDensityEstimation_tile(mgrid(M, tile), None)
+    _, o64, ne = orc.density_tile(x, y, w, ew, np.full(100, h, np.float32), tile, None)
+    assert_density_close(got, o64)
+    assert ctx.count_density_pairs(mgrid(M, tile), None) == ne
+
+
+def test_cutoff_edge_is_exclusive(ctx, orc):
+    """Integer points with h = 1: cells at distance exactly 3 are excluded
+    (d2 = 9 = 9*h2 is not < 9*h2), as the reference's strict test."""
+    import mgpukde_tiled_amd as M
+
+    x = np.array([10.5, 10.5, 20.5, 33.5], np.float32)
+    y = np.array([10.5, 10.5, 5.5, 33.5], np.float32)  # a duplicate point too
+    w = np.ones(4, np.float32)
+    tile = orc.grid(50, 50, 0.0, 0.0, 1.0)
+    ctx.points_upload(x, y, w)
+    ctx.set_bandwidth(1.0)
+    ctx.set_ec_weights(None)
+    got = ctx.KernelDensityEstimation_tile(mgrid(M, tile), None)
+    _, o64, ne = orc.density_tile(x, y, w, w, np.ones(4, np.float32), tile, None)
+    assert_density_close(got, o64)
+    # row of the point at y=10.5 is 39; the cell 3 columns right has d2 = 9: zero
+    assert got[39, 13] == 0.0 and got[39, 12] > 0.0
+    assert ctx.count_density_pairs(mgrid(M, tile), None) == ne
+
+
+@pytest.mark.parametrize("kind", ["uniform", "integer"])
+def test_range_candidates_exact(ctx, orc, kind):
+    r = np.random.default_rng(11)
+    n = 3000
+    if kind == "uniform":
+        x = r.uniform(0, 200, n).astype(np.float32)
+        y = r.uniform(0, 200, n).astype(np.float32)
+        h = r.uniform(1.0, 6.0, n).astype(np.float32)
+    else:
+        x = r.integers(0, 60, n).astype(np.float32)
+        y = r.integers(0, 60, n).astype(np.float32)
+        h = np.ones(n, np.float32)
+    ctx.points_upload(x, y, None)
+    ctx.set_bandwidths(h)
+    offs, idx = ctx.SearchRange_all()
+    o_offs, o_idx = orc.range_candidates(x, y, h)
+    assert np.array_equal(offs, o_offs)
+    assert np.array_equal(idx, o_idx)
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_density_at_points(ctx, orc, scene, adaptive):
+    s = scene
+    n = len(s["x"])
+    r = np.random.default_rng(13)
+    hs = r.uniform(5.0, 20.0, n).astype(np.float32) if adaptive else np.full(n, 9.0, np.float32)
+    ew = r.uniform(1.0, 1.5, n).astype(np.float32)
+    ctx.points_upload(s["x"], s["y"], s["w"])
+    ctx.set_bandwidths(hs)
+    ctx.set_ec_weights(ew)
+    d0 = ctx.DensityAtPoints(loo=False)
+    d1 = ctx.DensityAtPoints(loo=True)
+    o = orc.loo_density(s["x"], s["y"], s["w"], ew, hs)
+    assert rel_err(d0, o["den0_64"]).max() <= REL
+    pos = o["den1_64"] > 0
+    assert np.all(d1[~pos] == 0)
+    assert rel_err(d1[pos], o["den1_64"][pos]).max() <= REL
+
+
+def test_loo_loglik_batched(ctx, orc, scene):
+    import mgpukde_tiled_amd as M
+
+    s = scene
+    n = len(s["x"])
+    ctx.points_upload(s["x"], s["y"], s["w"])
+    ctx.mask_upload(mgrid(M, s["g"]), s["mask"])
+    hc = np.array([3.0, 5.5, 8.0, 12.0, 20.0], np.float32)
+    got = ctx.LogLikelihood(hc, edge_correction=True)
+    oflags, _ = orc.mark_boundary(s["g"], s["mask"])
+    od = orc.dist_to_boundary(s["x"], s["y"], s["g"], oflags)
+    for k, h in enumerate(hc):
+        hs = np.full(n, h, np.float32)
+        ew32, ew64, _ = orc.ec_weights(s["x"], s["y"], od, hs, s["g"], s["mask"])
+        o = orc.loo_density(s["x"], s["y"], s["w"], ew32, hs)
+        ref = orc.loglik_fp64(o["den1_64"])
+        assert abs(got[k] - ref) <= 1e-5 * n, (k, got[k], ref)
+
+
+def test_abramson(ctx, orc, scene):
+    import mgpukde_tiled_amd as M
+
+    s = scene
+    ctx.points_upload(s["x"], s["y"], s["w"])
+    ctx.mask_upload(mgrid(M, s["g"]), s["mask"])
+    h0 = ctx.BandWidth_rule_of_thumb()
+    assert h0 == orc.rule_of_thumb(s["x"], s["y"], s["w"])
+    hs, d0, g = ctx.Abramson(h0, -0.5, edge_correction=True)
+    n = len(s["x"])
+    oflags, _ = orc.mark_boundary(s["g"], s["mask"])
+    od = orc.dist_to_boundary(s["x"], s["y"], s["g"], oflags)
+    ew32, ew64, _ = orc.ec_weights(s["x"], s["y"], od, np.full(n, h0, np.float32), s["g"], s["mask"])
+    _, p64 = orc.pilot_density(s["x"], s["y"], s["w"], ew32, h0)
+    assert rel_err(d0, p64).max() <= REL
+    g64 = orc.compgml64(p64)
+    assert abs(g - g64) <= 1e-6 * g64
+    h64 = orc.abramson64(h0, -0.5, g64, p64)
+    assert rel_err(hs, h64).max() <= REL
