@@ -1,0 +1,237 @@
+#!/usr/bin/env python3
+"""Benchmark: 1M-point adaptive, edge-corrected KDE on an 8192x8192 raster.
+
+Workload (thomas_adaptive): 200 parents uniform in [0,8192)^2, Poisson(5000)
+children each with N(0, 40^2) offsets, clipped; weights U[0.5, 2].  Mask: disk
+of radius 3900 cells minus 20 rectangular holes of 50-400 cells, nodata -9999;
+the same raster is the edge-correction mask and the density mask (cs = 1).
+One step = one whole job on inputs resident in HBM: MarkBoundary +
+CalcDist2Boundary, EC(h0), pilot density at the rule-of-thumb h0, geometric
+mean, Abramson h_i = h0*(f_i/g)^-0.5, EC(h_i), the 8192^2 density raster.
+
+value = Gaussian kernel evaluations of the job (pairs inside the cut-off, all
+phases, counted exactly by separate untimed passes) / seconds per job.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+MI355X_FP32_TFLOPS = 157.3
+FLOP_PER_EVAL = 8  # 2 sub + 2 mul/fma for d^2, 1 mul for the scale, 1 fma to accumulate
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload(scale=1.0):
+    import cases
+
+    extent = 8192.0 * scale
+    n_parents = max(1, int(round(200 * scale * scale)))
+    x, y, w = cases.thomas_points(n_parents, 5000, 40.0, extent, seed=20260002)
+    side = int(round(8192 * scale))
+    mask = cases.disk_mask(side, side, 3900.0 * scale, n_holes=20, hole_min=max(2, int(50 * scale)),
+                           hole_max=max(3, int(400 * scale)), seed=20260002)
+    return x, y, w, mask, side
+
+
+def cpu_baseline(x, y, w, ew, hs, mask, side, target_s=15.0):
+    """The oracle's restatement of the reference CPU loop (brute force over all
+    points per cell, float accumulation) on a seeded sample of 64x64 tiles."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    rs = np.random.default_rng(20260099)
+    nthreads = O.num_threads()
+    times = 0.0
+    e_total = 0
+    cells = 0
+    tiles = 0
+    while times < target_s and tiles < 64:
+        r0 = int(rs.integers(0, side // 64)) * 64
+        c0 = int(rs.integers(0, side // 64)) * 64
+        tile = O.grid(64, 64, float(c0), float(side - r0 - 64), 1.0)
+        tm = mask[r0:r0 + 64, c0:c0 + 64]
+        t0 = time.perf_counter()
+        _, _, ne = O.density_tile(x, y, w, ew, hs, tile, tm, want64=False)
+        times += time.perf_counter() - t0
+        e_total += ne
+        cells += int(np.count_nonzero(tm != -9999.0))
+        tiles += 1
+    return {
+        "value": e_total / times if times > 0 else None,
+        "unit": "evals/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": "%d seeded 64x64 tiles of the 8192^2 raster (%d valid cells, %d evaluations), all %d points, "
+                  "reference CPU loop (brute force over all points per cell, float accumulation), %.1f s"
+                  % (tiles, cells, e_total, len(x), times),
+        "cells_per_s": cells / times if times > 0 else None,
+    }
+
+
+def accuracy(x, y, w, ew, hs, mask, side, den, ntiles=3):
+    """Max / mean relative error of the GPU raster against the FP64 oracle on a
+    few seeded 64x64 tiles."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    rs = np.random.default_rng(20260098)
+    mx, acc, cnt = 0.0, 0.0, 0
+    for _ in range(ntiles):
+        r0 = int(rs.integers(0, side // 64)) * 64
+        c0 = int(rs.integers(0, side // 64)) * 64
+        tile = O.grid(64, 64, float(c0), float(side - r0 - 64), 1.0)
+        tm = mask[r0:r0 + 64, c0:c0 + 64]
+        _, o64, _ = O.density_tile(x, y, w, ew, hs, tile, tm, want64=True)
+        g = den[r0:r0 + 64, c0:c0 + 64]
+        v = (tm != -9999.0) & (o64 > 0)
+        if np.any(v):
+            e = np.abs(g[v].astype(np.float64) - o64[v]) / o64[v]
+            mx = max(mx, float(e.max()))
+            acc += float(e.sum())
+            cnt += int(v.sum())
+    return {"max_rel_err": mx, "mean_rel_err": acc / max(cnt, 1), "cells_checked": cnt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (testing only)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import mgpukde_tiled_amd as M
+
+    log("workload: generating inputs")
+    x, y, w, mask, side = workload(args.scale)
+    n = len(x)
+    log("workload: %d points, raster %dx%d" % (n, side, side))
+    ctx = M.Context(0)
+    grid = M.grid(side, side, 0.0, 0.0, 1.0)
+    ctx.points_upload(x, y, w)
+    ctx.mask_upload(grid, mask)
+    ctx.tile_upload(grid, mask)
+
+    def step():
+        return ctx.run_job(hoption=2, edge_correction=True, optimise=False, alpha=-0.5, recompute_boundary=True)
+
+    for i in range(args.warmup):
+        st = step()
+        log("warmup %d: %.1f ms" % (i, st["ms_total"]))
+    ctx.synchronize()
+    wall = []
+    stats = []
+    t_all0 = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        st = step()
+        ctx.synchronize()
+        wall.append((time.perf_counter() - t0) * 1e3)
+        stats.append(st)
+        log("step %d: %.1f ms (device %.1f ms; density %.1f ms)" % (i, wall[-1], st["ms_total"], st["ms_density"]))
+    ctx.synchronize()
+    t_all = time.perf_counter() - t_all0
+    ms_per_step = t_all * 1e3 / args.steps
+    dens_ms = [ctx.last_kernel_ms("density")]
+
+    # --- exact work counts (untimed) ---------------------------------------
+    hs = ctx.get_bandwidths()
+    ew = ctx.get_ec_weights()
+    e_den = ctx.count_density_pairs(grid, mask)
+    e_ec = stats[-1]["n_ec_pairs"]
+    h0 = stats[-1]["h0"]
+    # pilot pairs and EC(h0) pairs at the rule-of-thumb bandwidth
+    ctx.set_bandwidth(h0)
+    _, e_ec0 = ctx.EdgeCorrectionWeightsExact()
+    offs, _ = ctx_range_total(ctx)
+    e_pilot = offs
+    ctx.set_bandwidths(hs)
+    ctx.set_ec_weights(ew)
+    e_total = e_den + e_ec + e_ec0 + e_pilot
+    log("work: E_den=%d E_ec=%d E_ec0=%d E_pilot=%d" % (e_den, e_ec, e_ec0, e_pilot))
+
+    dens_kernel_ms = float(np.median([s["ms_density"] for s in stats]))
+    ker_ms = dens_ms[0] if dens_ms[0] > 0 else dens_kernel_ms
+    achieved_tflops = FLOP_PER_EVAL * e_den / (ker_ms * 1e-3) / 1e12
+    valid_cells = int(np.count_nonzero(mask != -9999.0))
+
+    out = {
+        "metric": "Gaussian kernel evals/s + %VALU peak, 1M-pt adaptive KDE; p99 job latency",
+        "value": e_total / (ms_per_step * 1e-3),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "thomas_adaptive: %d Thomas-process points (200 parents, Poisson(5000), sigma 40), "
+                        "weights U[0.5,2], %dx%d raster cs 1, disk r=3900 minus 20 holes, Abramson alpha -0.5 "
+                        "at the rule-of-thumb h0, edge correction on" % (n, side, side),
+            "points": n,
+            "raster": [side, side],
+            "parallelism": "single GPU",
+        },
+        "latency_ms": {"p50": float(np.percentile(wall, 50)), "p99": float(np.percentile(wall, 99)),
+                       "max": float(np.max(wall))},
+        "work": {"E_density": e_den, "E_ec_final": e_ec, "E_ec_pilot": e_ec0, "E_pilot": e_pilot,
+                 "valid_cells": valid_cells},
+        "phases_ms": {k: float(np.median([s[k] for s in stats]))
+                      for k in ("ms_boundary", "ms_ec", "ms_bandwidth", "ms_density", "ms_total")},
+        "density_kernel": {"evals_per_s": e_den / (ker_ms * 1e-3), "cells_per_s": valid_cells / (ker_ms * 1e-3),
+                           "ms": ker_ms},
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved_tflops,
+            "peak": MI355X_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / MI355X_FP32_TFLOPS,
+            "traffic": None,
+            "kernel": "k_density_tile",
+            "work_per_launch": "%d evaluations x %d FLOP" % (e_den, FLOP_PER_EVAL),
+        },
+    }
+    if not args.no_accuracy:
+        den = ctx.tile_download()
+        out["accuracy_vs_fp64_oracle"] = accuracy(x, y, w, ew, hs, mask, side, den)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(x, y, w, ew, hs, mask, side)
+    else:
+        out["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def ctx_range_total(ctx):
+    import ctypes
+
+    import mgpukde_tiled_amd as M
+
+    tot = ctypes.c_int64()
+    M._check(M._lib.mkde_range_candidates(ctx._h, None, None, 0, ctypes.byref(tot)))
+    return tot.value, None
+
+
+if __name__ == "__main__":
+    main()

@@ -182,6 +182,43 @@ int mkde_tile_download(mkde_ctx *ctx, float *out);
  * (milliseconds, HIP events around the dominant kernel). */
 int mkde_last_kernel_ms(mkde_ctx *ctx, const char *which, double *ms);
 
+/* ------------------------------------------------------------------ jobs --
+ * One KDE job on the context's resident points, mask and tile: the sequence
+ * of the reference's GPU block in main() (tiled:877-1517) without any host
+ * staging.  Bandwidth options (tiled:280-284):
+ *   MKDE_H_FILE (-1)  use the bandwidths already set on the context
+ *   MKDE_H_RULE (0)   rule-of-thumb h0 for every point
+ *   MKDE_H_MLE  (1)   MLE_FixedBandWidth from h0
+ *   MKDE_H_ADAPTIVE (2) hooke_jeeves from (h0, alpha0 = -0.5) when optimise
+ *                     != 0, then Abramson bandwidths from the result; with
+ *                     optimise == 0 the Abramson step runs directly at
+ *                     (h0, alpha) -- the fixed-alpha adaptive estimator.
+ * Edge correction (UPDATEWEIGHTS) recomputes boundary distances when
+ * recompute_boundary != 0.  The density of the uploaded tile is left in the
+ * context (mkde_tile_download copies it out). */
+typedef struct {
+    int hoption;
+    int edge_correction;
+    int optimise;
+    float alpha;
+    int recompute_boundary;
+} mkde_job_opts;
+
+typedef struct {
+    float h0;              /* rule-of-thumb bandwidth */
+    float h_opt;           /* MLE result (hoption 1) or optimised h0 (hoption 2) */
+    float alpha_opt;       /* optimised alpha (hoption 2) */
+    double g;              /* geometric mean of the pilot densities (hoption 2) */
+    int64_t n_ec_pairs;    /* (point, valid cell) pairs of the last EC pass */
+    double ms_boundary;    /* MarkBoundary + CalcDist2Boundary */
+    double ms_ec;          /* all EC passes */
+    double ms_bandwidth;   /* bandwidth selection (pilot, optimisers) */
+    double ms_density;     /* density raster kernel */
+    double ms_total;       /* whole job, device time on the context stream */
+} mkde_job_stats;
+
+int mkde_run_job(mkde_ctx *ctx, const mkde_job_opts *opts, mkde_job_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

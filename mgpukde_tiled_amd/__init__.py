@@ -49,6 +49,38 @@ def grid(ncols, nrows, xll, yll, cellsize, nodata=-9999.0):
     return Grid(int(ncols), int(nrows), float(xll), float(yll), float(cellsize), float(nodata))
 
 
+class JobOpts(ctypes.Structure):
+    """mkde_job_opts (see include/mkde.h)."""
+
+    _fields_ = [
+        ("hoption", ctypes.c_int),
+        ("edge_correction", ctypes.c_int),
+        ("optimise", ctypes.c_int),
+        ("alpha", ctypes.c_float),
+        ("recompute_boundary", ctypes.c_int),
+    ]
+
+
+class JobStats(ctypes.Structure):
+    """mkde_job_stats (see include/mkde.h)."""
+
+    _fields_ = [
+        ("h0", ctypes.c_float),
+        ("h_opt", ctypes.c_float),
+        ("alpha_opt", ctypes.c_float),
+        ("g", ctypes.c_double),
+        ("n_ec_pairs", ctypes.c_int64),
+        ("ms_boundary", ctypes.c_double),
+        ("ms_ec", ctypes.c_double),
+        ("ms_bandwidth", ctypes.c_double),
+        ("ms_density", ctypes.c_double),
+        ("ms_total", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _FP = ctypes.POINTER(ctypes.c_float)
 _DP = ctypes.POINTER(ctypes.c_double)
 _I64P = ctypes.POINTER(ctypes.c_int64)
@@ -92,6 +124,7 @@ SIGNATURES = {
     "mkde_density_tile_resident": [_CTX],
     "mkde_tile_download": [_CTX, _FP],
     "mkde_last_kernel_ms": [_CTX, ctypes.c_char_p, _DP],
+    "mkde_run_job": [_CTX, ctypes.POINTER(JobOpts), ctypes.POINTER(JobStats)],
 }
 _RESTYPES = {"mkde_last_error": ctypes.c_char_p, "mkde_version": ctypes.c_char_p}
 
@@ -304,6 +337,15 @@ class Context:
         out = np.empty(t.ncols * t.nrows, np.float32)
         _check(_lib.mkde_tile_download(self._h, _p(out, _FP)))
         return out.reshape(t.nrows, t.ncols)
+
+    def run_job(self, hoption=2, edge_correction=True, optimise=True, alpha=-0.5, recompute_boundary=True):
+        """One KDE job on the resident points / mask / tile (the GPU block of
+        the reference's main()).  Returns the job statistics as a dict."""
+        o = JobOpts(int(hoption), 1 if edge_correction else 0, 1 if optimise else 0, float(alpha),
+                    1 if recompute_boundary else 0)
+        st = JobStats()
+        _check(_lib.mkde_run_job(self._h, ctypes.byref(o), ctypes.byref(st)))
+        return st.as_dict()
 
     def last_kernel_ms(self, which="density"):
         v = ctypes.c_double()

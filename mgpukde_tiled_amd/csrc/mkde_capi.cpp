@@ -1050,3 +1050,106 @@ extern "C" int mkde_last_kernel_ms(mkde_ctx *c, const char *which, double *ms)
     else return fail(MKDE_E_ARG, "unknown kernel '%s'", which);
     return MKDE_OK;
 }
+
+// ------------------------------------------------------------------ jobs --
+
+namespace {
+struct PhaseTimer {
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    double *acc;
+    explicit PhaseTimer(hipStream_t st, double *accum) : s(st), acc(accum)
+    {
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        (void)hipEventRecord(a, s);
+    }
+    ~PhaseTimer()
+    {
+        float ms = 0.0f;
+        (void)hipEventRecord(b, s);
+        if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess && acc) *acc += ms;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+};
+}  // namespace
+
+extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats *st)
+{
+    CHK(need_points(c));
+    if (!o) return fail(MKDE_E_ARG, "options are NULL");
+    if (o->hoption < -1 || o->hoption > 2) return fail(MKDE_E_ARG, "hoption %d not in {-1,0,1,2}", o->hoption);
+    if (!c->tile_set) return fail(MKDE_E_STATE, "no density tile uploaded");
+    const int ec_on = o->edge_correction ? 1 : 0;
+    if (ec_on) CHK(need_mask(c));
+    if (o->hoption == -1) CHK(need_h(c));
+    mkde_job_stats S;
+    memset(&S, 0, sizeof S);
+    hipEvent_t t0, t1;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    HIPCHK(hipEventRecord(t0, c->stream));
+    S.h0 = rule_of_thumb_host(c->hx, c->hy, c->hw);
+    if (ec_on) {
+        PhaseTimer pt(c->stream, &S.ms_boundary);
+        if (o->recompute_boundary) {
+            c->nboundary = -1;
+            c->dist_set = false;
+        }
+        CHK(ensure_dist(c));
+    }
+    if (o->hoption != -1) CHK(mkde_set_bandwidth_fixed(c, S.h0));
+    unsigned long long ne = 0;
+    auto ec_pass = [&](void) -> int {
+        PhaseTimer pt(c->stream, &S.ms_ec);
+        int s = run_ec(c, c->h.p, c->h_fixed, c->ew.p, &ne);
+        c->pts_dirty = true;
+        return s;
+    };
+    if (ec_on) CHK(ec_pass());
+    else HIPCHK(mkde_dev::fill_f32(c->stream, c->n, 1.0f, c->ew.p));
+    c->pts_dirty = true;
+    if (o->hoption == 1) {
+        {
+            PhaseTimer pt(c->stream, &S.ms_bandwidth);
+            int it = 0;
+            CHK(mkde_mle_fixed_bandwidth(c, S.h0, ec_on, &S.h_opt, &it));
+        }
+        CHK(mkde_set_bandwidth_fixed(c, S.h_opt));
+        if (ec_on) CHK(ec_pass());
+    } else if (o->hoption == 2) {
+        float h0 = S.h0, a0 = o->optimise ? -0.5f : o->alpha;
+        {
+            PhaseTimer pt(c->stream, &S.ms_bandwidth);
+            if (o->optimise) {
+                float out3[3];
+                int it = 0;
+                CHK(mkde_hooke_jeeves(c, h0, -0.5f, h0 / 10, 0.1f, ec_on, out3, &it));
+                h0 = out3[0];
+                a0 = out3[1];
+            }
+            // EC(h0) inside, pilot, g, h_i
+            CHK(abramson_core(c, h0, a0, ec_on));
+        }
+        S.h_opt = h0;
+        S.alpha_opt = a0;
+        CHK(copy_d2h(c, &S.g, c->gval.p, sizeof(double)));
+        if (ec_on) CHK(ec_pass());
+    }
+    S.n_ec_pairs = (int64_t)ne;
+    {
+        PhaseTimer pt(c->stream, &S.ms_density);
+        CHK(mkde_density_tile_resident(c));
+    }
+    HIPCHK(hipEventRecord(t1, c->stream));
+    HIPCHK(hipEventSynchronize(t1));
+    float tot = 0.0f;
+    HIPCHK(hipEventElapsedTime(&tot, t0, t1));
+    HIPCHK(hipEventElapsedTime(&c->last_density_ms, c->ev0, c->ev1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    S.ms_total = tot;
+    if (st) *st = S;
+    return MKDE_OK;
+}

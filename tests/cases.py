@@ -62,9 +62,10 @@ def mixture_points(n, extent, n_centres=64, sigma_frac=0.02, seed=20260003):
 def disk_mask(ncols, nrows, radius_cells, n_holes=0, hole_min=50, hole_max=400, seed=20260002, nodata=NODATA):
     """Disk of valid cells (value 1.0) centred on the raster, minus holes."""
     r = rng(seed + 1000)
-    rr, cc = np.mgrid[0:nrows, 0:ncols]
+    rr, cc = np.ogrid[0:nrows, 0:ncols]
     cy, cx = (nrows - 1) / 2.0, (ncols - 1) / 2.0
-    m = np.where((rr - cy) ** 2 + (cc - cx) ** 2 <= radius_cells ** 2, 1.0, nodata).astype(np.float32)
+    m = np.full((nrows, ncols), nodata, np.float32)
+    m[(rr - cy) ** 2 + (cc - cx) ** 2 <= radius_cells ** 2] = 1.0
     for _ in range(n_holes):
         hw = int(r.integers(hole_min, hole_max + 1))
         hh = int(r.integers(hole_min, hole_max + 1))
