@@ -1,0 +1,437 @@
+"""Pinning the oracle (CPU, no GPU).
+
+The oracle restates the reference CPU path; these tests check it against what
+does not depend on it: closed forms, invariants, hand-written expected values,
+the reference's own k-d tree (compiled from its sources into oracle/_ref) and
+independent numpy brute force on small inputs.  Each check is chosen so that a
+plausible mistake (a dropped factor, a wrong sign or index, a transposed
+operand, a non-strict cut-off) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cases
+
+TWO_PI_F = float(np.float32(6.2831853071795862))
+
+
+# ------------------------------------------------------------ helpers ----
+def np_centres(g):
+    c = np.arange(g.ncols, dtype=np.float64)
+    r = np.arange(g.nrows, dtype=np.float64)
+    cx = (np.float64(g.xll) + (c + 0.5) * np.float64(g.cs)).astype(np.float32)
+    cy = (np.float64(g.yll) + ((np.float64(g.nrows) - 0.5) - r) * np.float64(g.cs)).astype(np.float32)
+    return cx, cy
+
+
+def np_d2(px, py, cx, cy):
+    dx = (np.float32(cx) - np.float32(px)).astype(np.float32).astype(np.float64)
+    dy = (np.float32(cy) - np.float32(py)).astype(np.float32).astype(np.float64)
+    return dx * dx + dy * dy
+
+
+def np_density64(x, y, w, ew, h, g, valid=None):
+    cx, cy = np_centres(g)
+    out = np.zeros((g.nrows, g.ncols))
+    cnt = 0
+    for i in range(len(x)):
+        h2 = np.float64(np.float32(h[i]) * np.float32(h[i]))
+        d2 = np_d2(x[i], y[i], cx[None, :], cy[:, None])
+        m = d2 < 9.0 * h2
+        if valid is not None:
+            m &= valid
+        out += np.where(m, np.exp(-d2 / (2 * h2)) / (h2 * TWO_PI_F), 0.0) * np.float64(w[i]) * np.float64(ew[i])
+        cnt += int(m.sum())
+    return out, cnt
+
+
+# ---------------------------------------------------------------- N2 ----
+def test_cell_centres_exact_grid(orc):
+    g = orc.grid(7, 5, 10.0, 20.0, 1.0)
+    assert orc.col_x(g, 0) == 10.5 and orc.col_x(g, 6) == 16.5
+    assert orc.row_y(g, 0) == 24.5  # row 0 is the north row
+    assert orc.row_y(g, 4) == 20.5
+
+
+def test_cell_centres_redwood_geometry(orc):
+    # the Redwood rasters: 128x128, cellsize 1/128, origin (0, 1) top-left
+    g = orc.grid(128, 128, 0.0, float(1.0 + 128 * (-0.0078125)), 0.0078125)
+    for c in (0, 17, 127):
+        assert orc.col_x(g, c) == np.float32((c + 0.5) / 128)
+    for r in (0, 64, 127):
+        assert orc.row_y(g, r) == np.float32(1.0 - (r + 0.5) / 128)
+
+
+# ---------------------------------------------------------------- a3 ----
+def test_kernel_closed_form(orc):
+    h = 3.7
+    h2 = float(np.float32(np.float32(h) * np.float32(h)))
+    assert orc.kernel(h2, 0.0) == np.float32(1.0 / (h2 * TWO_PI_F))
+    for d2 in (0.5, 4.0, 30.0, 9 * h2 * 0.999):
+        ref = math.exp(-d2 / (2 * h2)) / (2 * math.pi * h2)
+        assert abs(orc.kernel(h2, d2) - ref) <= 2e-7 * ref
+        assert abs(orc.kernel64(h2, d2) - ref) <= 1e-7 * ref  # only TWO_PI's float rounding
+    # strict cut-off at d^2 = 9 h^2
+    assert orc.kernel(h2, 9 * h2) == 0.0
+    assert orc.kernel64(h2, 9 * h2) == 0.0
+    assert orc.kernel(h2, np.nextafter(9 * h2, 0)) > 0.0
+
+
+def test_distance_float_subtraction(orc):
+    # dx is the float difference: 1e8 + 1 is not representable in float
+    assert orc.distance2(0.0, 0.0, 3.0, 4.0) == 25.0
+    assert orc.distance2(1e8, 0.0, 1e8 + 1.0, 0.0) == 0.0
+
+
+# ---------------------------------------------------------------- N9 ----
+def test_rule_of_thumb_closed_form(orc):
+    a, c = 3.0, 50.0
+    x = np.array([c - a, c + a, c - a, c + a], np.float32)
+    y = np.array([c - a, c - a, c + a, c + a], np.float32)
+    w = np.ones(4, np.float32)
+    # mean centre (c, c); D2 = 2 a^2; h0 = sqrt(D2) * (2/(3n))^(1/4)
+    ref = math.sqrt(2 * a * a) * (2.0 / 12.0) ** 0.25
+    h = orc.rule_of_thumb(x, y, w)
+    assert abs(h - ref) <= 2e-7 * ref
+
+
+def test_rule_of_thumb_weights_equal_duplicates(orc):
+    r = np.random.default_rng(1)
+    x = r.uniform(0, 10, 50).astype(np.float32)
+    y = r.uniform(0, 10, 50).astype(np.float32)
+    w = np.where(np.arange(50) < 10, 2.0, 1.0).astype(np.float32)
+    x2 = np.concatenate([x, x[:10]])
+    y2 = np.concatenate([y, y[:10]])
+    h_w = orc.rule_of_thumb(x, y, w)
+    # weighted: sum of weights is 60; duplicating the 10 weight-2 points gives
+    # the same moments, but n differs (50 vs 60) in (2/(3n))^(1/4)
+    h_d = orc.rule_of_thumb(x2, y2, np.ones(60, np.float32))
+    assert abs(h_w / h_d - (60 / 50) ** 0.25) <= 1e-6
+
+
+# ---------------------------------------------------------------- N5 ----
+N = cases.NODATA
+
+
+def test_mark_boundary_hand_written(orc):
+    m5 = np.zeros((5, 5), np.float32)
+    flags, nb = orc.mark_boundary(orc.grid(5, 5, 0, 0, 1), m5)
+    exp = np.ones((5, 5), np.uint8)
+    exp[1:4, 1:4] = 0
+    assert np.array_equal(flags, exp) and nb == 16
+
+    m5h = m5.copy()
+    m5h[2, 2] = N
+    flags, nb = orc.mark_boundary(orc.grid(5, 5, 0, 0, 1), m5h)
+    exp = np.ones((5, 5), np.uint8)
+    exp[2, 2] = 0  # nodata is never a boundary cell
+    assert np.array_equal(flags, exp) and nb == 24
+
+    L = np.array([
+        [1, 1, N, N, N, N],
+        [1, 1, N, N, N, N],
+        [1, 1, 1, 1, 1, 1],
+        [1, 1, 1, 1, 1, 1],
+        [1, 1, 1, 1, 1, 1],
+        [1, 1, 1, 1, 1, 1],
+    ], np.float32)
+    expL = np.array([
+        [1, 1, 0, 0, 0, 0],
+        [1, 1, 0, 0, 0, 0],
+        [1, 1, 1, 1, 1, 1],   # row 2 touches the nodata block above (diagonals too)
+        [1, 0, 0, 0, 0, 1],
+        [1, 0, 0, 0, 0, 1],
+        [1, 1, 1, 1, 1, 1],
+    ], np.uint8)
+    flags, nb = orc.mark_boundary(orc.grid(6, 6, 0, 0, 1), L)
+    assert np.array_equal(flags, expL) and nb == int(expL.sum())
+
+
+# ---------------------------------------------------------------- N6 ----
+def test_dist_to_boundary_hand_and_brute(orc):
+    g = orc.grid(5, 5, 0.0, 0.0, 1.0)
+    flags, _ = orc.mark_boundary(g, np.zeros((5, 5), np.float32))
+    d = orc.dist_to_boundary(np.array([2.5, -1.0], np.float32), np.array([2.5, 2.5], np.float32), g, flags)
+    assert d[0] == 4.0 and d[1] == 2.25
+    # no boundary cell at all -> DBL_MAX
+    d = orc.dist_to_boundary(np.array([1.0], np.float32), np.array([1.0], np.float32), g,
+                             np.zeros((5, 5), np.uint8))
+    assert d[0] == np.finfo(np.float64).max
+    # brute force on a holed mask
+    m = cases.disk_mask(40, 30, 12, n_holes=2, hole_min=3, hole_max=6, seed=3)
+    g = orc.grid(40, 30, 5.0, -3.0, 0.5)
+    flags, _ = orc.mark_boundary(g, m)
+    r = np.random.default_rng(2)
+    px = r.uniform(0, 30, 200).astype(np.float32)
+    py = r.uniform(-5, 15, 200).astype(np.float32)
+    cx, cy = np_centres(g)
+    rr, cc = np.nonzero(flags)
+    ref = np.array([np_d2(px[i], py[i], cx[cc], cy[rr]).min() for i in range(200)])
+    assert np.array_equal(orc.dist_to_boundary(px, py, g, flags), ref)
+
+
+# ---------------------------------------------------------------- N4 ----
+def test_ec_closed_form_interior_point(orc):
+    # 61x61 valid cells, point at the centre, h slightly above 10 so that the
+    # boundary ring (distance 30) is inside the cut-off: the point is not
+    # skipped and its disk lies inside the mask.  The EC sum is then the
+    # discretised mass of the truncated Gaussian, 1 - exp(-4.5).
+    g = orc.grid(61, 61, 0.0, 0.0, 1.0)
+    mask = np.zeros((61, 61), np.float32)
+    x = np.array([30.5], np.float32)
+    y = np.array([30.5], np.float32)
+    flags, _ = orc.mark_boundary(g, mask)
+    d = orc.dist_to_boundary(x, y, g, flags)
+    h = np.array([10.01], np.float32)
+    assert d[0] < 9 * float(np.float32(h[0] * h[0]))
+    ew32, ew64, ne = orc.ec_weights(x, y, d, h, g, mask)
+    assert abs(1.0 / ew64[0] - (1 - math.exp(-4.5))) <= 1e-3
+    assert abs(ew32[0] - ew64[0]) <= 1e-5 * ew64[0]
+    # skipped point (dist >= 9 h^2) gets weight exactly 1
+    ew32, ew64, _ = orc.ec_weights(x, y, d, np.array([10.0], np.float32), g, mask)
+    assert ew32[0] == 1.0 and ew64[0] == 1.0
+
+
+def test_ec_half_plane(orc):
+    # valid only for columns >= 100; point on the edge line x = 100, far from
+    # every other edge: about half of the truncated mass
+    g = orc.grid(200, 200, 0.0, 0.0, 1.0)
+    mask = np.zeros((200, 200), np.float32)
+    mask[:, :100] = N
+    x = np.array([100.0], np.float32)
+    y = np.array([100.0], np.float32)
+    flags, _ = orc.mark_boundary(g, mask)
+    d = orc.dist_to_boundary(x, y, g, flags)
+    _, ew64, _ = orc.ec_weights(x, y, d, np.array([12.0], np.float32), g, mask)
+    assert abs(1.0 / ew64[0] - 0.5 * (1 - math.exp(-4.5))) <= 2e-3
+
+
+def test_ec_matches_full_raster_brute_force(orc):
+    # the NARROW box is a superset of the cut-off disk: a scan of every valid
+    # cell gives the same sum and count
+    m = cases.disk_mask(60, 50, 22, n_holes=3, hole_min=3, hole_max=8, seed=9)
+    g = orc.grid(60, 50, -2.0, 7.0, 0.75)
+    flags, _ = orc.mark_boundary(g, m)
+    r = np.random.default_rng(4)
+    x = r.uniform(-5, 45, 150).astype(np.float32)
+    y = r.uniform(5, 47, 150).astype(np.float32)
+    h = r.uniform(0.5, 6.0, 150).astype(np.float32)
+    d = orc.dist_to_boundary(x, y, g, flags)
+    ew32, ew64, ne = orc.ec_weights(x, y, d, h, g, m)
+    cx, cy = np_centres(g)
+    valid = m != N
+    ca = np.float64(np.float32(g.cs * g.cs))
+    tot = 0
+    for i in range(150):
+        h2 = np.float64(np.float32(h[i] * h[i]))
+        if d[i] >= 9 * h2:
+            assert ew64[i] == 1.0
+            continue
+        d2 = np_d2(x[i], y[i], cx[None, :], cy[:, None])
+        sel = valid & (d2 < 9 * h2)
+        s = (np.exp(-d2[sel] / (2 * h2)) / (h2 * TWO_PI_F) * ca).sum()
+        tot += int(sel.sum())
+        assert abs(ew64[i] - 1.0 / s) <= 1e-12 / s
+    assert ne == tot
+    fin = np.isfinite(ew64)
+    assert np.max(np.abs(ew32[fin] - ew64[fin]) / ew64[fin]) <= 1e-5
+
+
+def test_narrow_box_emulates_size_t_semantics(orc):
+    g = orc.grid(50, 40, 0.0, 0.0, 1.0)
+    # inside: row_lower = 39 - int(26) = 13 (its centre y = 26.5 is 0.5 beyond
+    # py + 3h), row_upper = 39 - int(14) = 25, cols int(19) .. int(31)
+    assert orc.narrow_box(g, 25.0, 20.0, 2.0) == (13, 25, 19, 31)
+    # disk entirely above the raster: row_upper wraps to nrows-1, row_lower to 0
+    rl, ru, cl, cu = orc.narrow_box(g, 25.0, 100.0, 2.0)
+    assert (rl, ru) == (0, 39)
+
+
+# ---------------------------------------------------------------- N3 ----
+def test_density_single_point_closed_form(orc):
+    g = orc.grid(80, 60, 0.0, 0.0, 0.5)
+    x = np.array([17.3], np.float32)
+    y = np.array([12.9], np.float32)
+    w = np.array([1.7], np.float32)
+    ew = np.array([1.25], np.float32)
+    h = np.array([2.2], np.float32)
+    o32, o64, ne = orc.density_tile(x, y, w, ew, h, g)
+    cx, cy = np_centres(g)
+    d2 = np_d2(x[0], y[0], cx[None, :], cy[:, None])
+    h2 = np.float64(np.float32(h[0] * h[0]))
+    ref = np.where(d2 < 9 * h2, 1.7 * 1.25 * np.exp(-d2 / (2 * h2)) / (2 * math.pi * h2), 0.0)
+    nz = ref > 0
+    assert np.array_equal(o64 > 0, nz) and ne == int(nz.sum())
+    assert np.max(np.abs(o64[nz] - ref[nz]) / ref[nz]) <= 1e-7
+    assert np.max(np.abs(o32[nz] - ref[nz]) / ref[nz]) <= 1e-6
+
+
+def test_density_mass_invariant_with_edge_correction(orc):
+    # EC grid == density grid and every point within 3h of the boundary:
+    # each point carries mass w_i * ew_i * m_i = w_i, so sum(den) * cs^2 = sum(w)
+    g = orc.grid(40, 40, 0.0, 0.0, 1.0)
+    mask = np.zeros((40, 40), np.float32)
+    r = np.random.default_rng(8)
+    x = r.uniform(0, 40, 60).astype(np.float32)
+    y = r.uniform(0, 40, 60).astype(np.float32)
+    w = r.uniform(0.5, 2, 60).astype(np.float32)
+    h = np.full(60, 7.0, np.float32)
+    flags, _ = orc.mark_boundary(g, mask)
+    d = orc.dist_to_boundary(x, y, g, flags)
+    assert np.all(d < 9 * 49)
+    ew32, ew64, _ = orc.ec_weights(x, y, d, h, g, mask)
+    _, o64, _ = orc.density_tile(x, y, w, ew64.astype(np.float32), h, g, mask)
+    assert abs(o64.sum() - w.astype(np.float64).sum()) <= 1e-6 * w.sum()
+
+
+def test_density_brute_force_and_nodata(orc):
+    r = np.random.default_rng(21)
+    n = 40
+    x = r.uniform(0, 20, n).astype(np.float32)
+    y = r.uniform(0, 20, n).astype(np.float32)
+    w = r.uniform(0.5, 2, n).astype(np.float32)
+    ew = r.uniform(1, 2, n).astype(np.float32)
+    h = r.uniform(0.5, 3, n).astype(np.float32)
+    g = orc.grid(45, 33, 1.0, 2.0, 0.4)
+    m = np.zeros((33, 45), np.float32)
+    m[5:9, 10:30] = N
+    o32, o64, ne = orc.density_tile(x, y, w, ew, h, g, m)
+    ref, cnt = np_density64(x, y, w, ew, h, g, valid=(m != N))
+    v = m != N
+    assert np.all(o32[~v] == N) and np.all(o64[~v] == N)
+    assert np.allclose(o64[v], ref[v], rtol=1e-12, atol=0)
+    nz = ref[v] > 0
+    assert np.max(np.abs(o32[v][nz] - ref[v][nz]) / ref[v][nz]) <= 1e-5
+    assert ne == cnt
+
+
+# ---------------------------------------------------------------- N1 ----
+@pytest.mark.parametrize("kind", ["uniform", "integer", "duplicates"])
+def test_range_candidates_match_reference_kdtree(orc, kind):
+    if not orc.ref_kdtree_available():
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    r = np.random.default_rng(31)
+    n = 1500
+    if kind == "uniform":
+        x = r.uniform(0, 100, n).astype(np.float32)
+        y = r.uniform(0, 100, n).astype(np.float32)
+        h = r.uniform(0.5, 4, n).astype(np.float32)
+    elif kind == "integer":
+        x = r.integers(0, 40, n).astype(np.float32)
+        y = r.integers(0, 40, n).astype(np.float32)
+        h = np.ones(n, np.float32)
+    else:
+        x = np.repeat(r.uniform(0, 30, n // 3), 3).astype(np.float32)
+        y = np.repeat(r.uniform(0, 30, n // 3), 3).astype(np.float32)
+        h = np.full(len(x), 1.3, np.float32)
+    o1, i1 = orc.range_candidates(x, y, h)
+    o2, i2 = orc.ref_kdtree_range(x, y, h)
+    assert np.array_equal(o1, o2) and np.array_equal(i1, i2)
+
+
+def test_range_candidates_semantics(orc):
+    # self and duplicates included, d^2 = 9 h^2 excluded
+    x = np.array([0, 0, 3, 2], np.float32)
+    y = np.array([0, 0, 0, 2], np.float32)
+    o, i = orc.range_candidates(x, y, np.ones(4, np.float32))
+    assert list(i[o[0]:o[1]]) == [0, 1, 3]  # (3,0) is at distance exactly 3
+
+
+# ---------------------------------------------------------------- N7 ----
+def test_loo_identities_and_brute_force(orc):
+    r = np.random.default_rng(41)
+    n = 300
+    x = r.uniform(0, 30, n).astype(np.float32)
+    y = r.uniform(0, 30, n).astype(np.float32)
+    w = r.uniform(0.5, 2, n).astype(np.float32)
+    ew = r.uniform(1, 1.5, n).astype(np.float32)
+    for h in (np.full(n, 2.5, np.float32), r.uniform(1, 4, n).astype(np.float32)):
+        o = orc.loo_density(x, y, w, ew, h)
+        h2 = (h * h).astype(np.float64)
+        self_t = w.astype(np.float64) * ew / (h2 * TWO_PI_F)
+        assert np.allclose(o["den0_64"] - o["den1_64"], self_t, rtol=1e-9, atol=0)
+        # brute force: source i uses its own bandwidth (sample-point estimator)
+        d2 = np_d2(x[None, :], y[None, :], x[:, None], y[:, None])  # [j, i]
+        K = np.where(d2 < 9 * h2[None, :], np.exp(-d2 / (2 * h2[None, :])) / (h2[None, :] * TWO_PI_F), 0.0)
+        np.fill_diagonal(K, 0.0)
+        ref = (K * (w.astype(np.float64) * ew.astype(np.float64))[None, :]).sum(axis=1)
+        assert np.allclose(o["den1_64"], ref, rtol=1e-12, atol=1e-300)
+        pos = ref > 0
+        assert np.max(np.abs(o["den1"][pos] - ref[pos]) / ref[pos]) <= 1e-4
+        assert abs(orc.loglik_ref32(o["den1"]) - orc.loglik_fp64(o["den1_64"])) <= 1e-4 * abs(
+            orc.loglik_fp64(o["den1_64"]))
+
+
+def test_loglik_clamps_at_epsilon(orc):
+    eps = np.float32(1e-14)
+    assert orc.loglik_fp64(np.array([0.0, 1.0])) == math.log(float(eps))
+    assert orc.loglik_ref32(np.array([0.0, 1.0], np.float32)) == np.float32(np.log(eps))
+
+
+# ---------------------------------------------------------------- N8 ----
+def test_pilot_matches_loo_den0_fixed_h(orc):
+    r = np.random.default_rng(51)
+    n = 400
+    x = r.uniform(0, 50, n).astype(np.float32)
+    y = r.uniform(0, 50, n).astype(np.float32)
+    w = r.uniform(0.5, 2, n).astype(np.float32)
+    ew = r.uniform(1, 1.5, n).astype(np.float32)
+    d0, d0_64 = orc.pilot_density(x, y, w, ew, 3.3)
+    o = orc.loo_density(x, y, w, ew, np.full(n, 3.3, np.float32))
+    assert np.allclose(d0_64, o["den0_64"], rtol=1e-12, atol=0)
+    assert np.max(np.abs(d0 - d0_64) / d0_64) <= 1e-5
+
+
+def test_gml_and_abramson_closed_forms(orc):
+    assert abs(orc.compgml(np.array([2.0, 8.0], np.float32)) - 4.0) <= 1e-6
+    assert abs(orc.compgml64(np.array([2.0, 8.0])) - 4.0) <= 1e-12
+    den = np.full(10, 0.37, np.float32)
+    g = orc.compgml(den)
+    h = orc.abramson(5.0, -0.5, g, den)
+    assert np.allclose(h, 5.0, rtol=1e-6)
+    h = orc.abramson64(5.0, -0.5, 2.0, np.array([8.0, 0.5]))
+    assert np.allclose(h, [2.5, 10.0], rtol=1e-15)
+
+
+# --------------------------------------------------------------- N10 ----
+def test_golden_section_known_maximiser(orc):
+    hstar = 7.3
+    hopt, it, probes = orc.golden_section(10.0, lambda h: -(h - hstar) ** 2)
+    # width shrinks by 1 - 1/(1+sqrt 5) per iteration: 13 iterations to 1 %
+    assert it == 13
+    assert abs(hopt - hstar) <= (20.0 - 0.5) / 100
+    assert np.all(probes[:, 0] < probes[:, 1])
+
+
+def test_hooke_jeeves_known_maximiser(orc):
+    hs, as_ = 4.2, -0.3
+    (h, a, L), it = orc.hooke_jeeves(5.0, -0.5, 0.5, 0.1, lambda h, a: -(h - hs) ** 2 - 100 * (a - as_) ** 2)
+    assert abs(h - hs) <= 0.06 and abs(a - as_) <= 0.02
+    assert L <= 0.0 and it <= 31
+
+
+# --------------------------------------------------------------- N11 ----
+def test_tile_point_filter_is_superset(orc):
+    r = np.random.default_rng(61)
+    n = 500
+    x = r.uniform(-20, 120, n).astype(np.float32)
+    y = r.uniform(-20, 120, n).astype(np.float32)
+    h = r.uniform(1, 8, n).astype(np.float32)
+    tile = orc.grid(30, 25, 40.0, 50.0, 1.0)
+    keep = set(orc.tile_point_filter(x, y, h, tile).tolist())
+    for i in range(n):
+        _, _, ne = orc.density_tile(x[i:i + 1], y[i:i + 1], np.ones(1, np.float32), np.ones(1, np.float32),
+                                    h[i:i + 1], tile, want64=False)
+        if ne > 0:
+            assert i in keep
+
+
+# --------------------------------------------------------------- N13 ----
+def test_msvc_rand_known_sequence(orc):
+    _, _, raw = orc.msvc_rand_points(5, seed=1)
+    assert list(raw) == [41, 18467, 6334, 26500, 19169, 15724, 11478, 29358, 26962, 24464]
+    x, y, raw = orc.msvc_rand_points(100, seed=100)
+    assert x.min() >= 0 and x.max() <= 100
+    assert x[0] == np.float32(np.float32(raw[0]) * np.float32(100.0) / np.float32(32767))
