@@ -111,6 +111,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (testing only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--stats", action="store_true", help="report density-kernel classification counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -150,6 +151,13 @@ def main():
     t_all = time.perf_counter() - t_all0
     ms_per_step = t_all * 1e3 / args.steps
     dens_ms = [ctx.last_kernel_ms("density")]
+
+    dstats = None
+    if args.stats:
+        ctx.density_stats(True)
+        step()
+        dstats = ctx.density_stats(False)
+        log("density stats: %s" % dstats)
 
     # --- exact work counts (untimed) ---------------------------------------
     hs = ctx.get_bandwidths()
@@ -212,6 +220,8 @@ def main():
             "work_per_launch": "%d evaluations x %d FLOP" % (e_den, FLOP_PER_EVAL),
         },
     }
+    if dstats is not None:
+        out["density_kernel"]["classification"] = dstats
     if not args.no_accuracy:
         den = ctx.tile_download()
         out["accuracy_vs_fp64_oracle"] = accuracy(x, y, w, ew, hs, mask, side, den)

@@ -178,6 +178,14 @@ int mkde_tile_upload(mkde_ctx *ctx, const mkde_grid *tile, const float *mask);
 int mkde_density_tile_resident(mkde_ctx *ctx);
 int mkde_tile_download(mkde_ctx *ctx, float *out);
 
+/* Diagnostic counters of the density kernel (no reference counterpart).
+ * enable != 0 zeroes and enables them; counters (8 entries, may be NULL)
+ * receives: [0] candidate points classified, [1] FULL and [2] PARTIAL
+ * (point, 64x64 block) pairs, [3] FULL, [4] PARTIAL and [5] EMPTY (point,
+ * 64x16 wave sub-block) pairs of PARTIAL points, [6] blocks without a valid
+ * cell; enable == 0 reads them and switches them off. */
+int mkde_density_stats(mkde_ctx *ctx, int enable, int64_t *counters);
+
 /* Kernel timing of the last density launch set on the context's stream
  * (milliseconds, HIP events around the dominant kernel). */
 int mkde_last_kernel_ms(mkde_ctx *ctx, const char *which, double *ms);

@@ -125,6 +125,7 @@ SIGNATURES = {
     "mkde_tile_download": [_CTX, _FP],
     "mkde_last_kernel_ms": [_CTX, ctypes.c_char_p, _DP],
     "mkde_run_job": [_CTX, ctypes.POINTER(JobOpts), ctypes.POINTER(JobStats)],
+    "mkde_density_stats": [_CTX, ctypes.c_int, _I64P],
 }
 _RESTYPES = {"mkde_last_error": ctypes.c_char_p, "mkde_version": ctypes.c_char_p}
 
@@ -346,6 +347,13 @@ class Context:
         st = JobStats()
         _check(_lib.mkde_run_job(self._h, ctypes.byref(o), ctypes.byref(st)))
         return st.as_dict()
+
+    def density_stats(self, enable):
+        out = np.zeros(8, np.int64)
+        _check(_lib.mkde_density_stats(self._h, 1 if enable else 0, _p(out, _I64P)))
+        keys = ["candidates", "full_blocks", "partial_blocks", "full_waves", "partial_waves", "empty_waves",
+                "skipped_blocks", "unused"]
+        return dict(zip(keys, out.tolist()))
 
     def last_kernel_ms(self, which="density"):
         v = ctypes.c_double()

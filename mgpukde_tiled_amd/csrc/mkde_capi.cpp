@@ -25,7 +25,8 @@ namespace mkde_dev {
 hipError_t prep_points(hipStream_t, int64_t, const int *, const float *, const float *, const float *, const float *,
                        const float *, float, PtRec *);
 hipError_t bin_stats(hipStream_t, int, const int *, const PtRec *, BinInfo *, double *);
-hipError_t density_tile(hipStream_t, const GridDesc &, const float *, float *, const PointIndex &);
+hipError_t density_tile(hipStream_t, const GridDesc &, const float *, float *, const PointIndex &,
+                        unsigned long long *);
 hipError_t point_density(hipStream_t, const PointIndex &, int, float *);
 hipError_t loo_multi(hipStream_t, const PointIndex &, const float *, const double *, const double *, int, float *);
 hipError_t range_count(hipStream_t, const PointIndex &, int64_t *);
@@ -167,6 +168,9 @@ struct mkde_ctx {
     DevBuf<double> partial, dsum, gval;
     DevBuf<float> coefk, ewk, den1k;
     DevBuf<double> ak, tk;
+    // density kernel counters (mkde_density_stats)
+    bool dt_stats_on = false;
+    DevBuf<unsigned long long> dt_stats;
 };
 
 extern "C" const char *mkde_last_error(void) { return g_err.c_str(); }
@@ -975,7 +979,8 @@ extern "C" int mkde_density_tile_resident(mkde_ctx *c)
     CHK(prep(c));
     PointIndex I = index_of(c);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(mkde_dev::density_tile(c->stream, c->tg, c->tile_has_mask ? c->tmask.p : nullptr, c->tout.p, I));
+    HIPCHK(mkde_dev::density_tile(c->stream, c->tg, c->tile_has_mask ? c->tmask.p : nullptr, c->tout.p, I,
+                                  c->dt_stats_on ? c->dt_stats.p : nullptr));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     return MKDE_OK;
 }
@@ -1038,6 +1043,22 @@ extern "C" int mkde_count_density_pairs(mkde_ctx *c, const mkde_grid *tile, cons
     unsigned long long t = 0;
     CHK(copy_d2h(c, &t, c->ucount.p, sizeof t));
     *n_pairs = (int64_t)t;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_density_stats(mkde_ctx *c, int enable, int64_t *counters)
+{
+    if (!c) return fail(MKDE_E_ARG, "null context");
+    CHK(set_device(c));
+    CHK(c->dt_stats.ensure(8));
+    if (enable) {
+        HIPCHK(hipMemsetAsync(c->dt_stats.p, 0, 8 * sizeof(unsigned long long), c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->dt_stats_on = true;
+    } else {
+        c->dt_stats_on = false;
+    }
+    if (counters) CHK(copy_d2h(c, counters, c->dt_stats.p, 8 * sizeof(int64_t)));
     return MKDE_OK;
 }
 

@@ -229,15 +229,18 @@ __global__ void k_bin_stats(int nbins, const int *bin_start, const PtRec *pts, B
 //
 // One 256-thread workgroup computes a 64x64-cell block of a tile.  Lane l of
 // wave w owns column c0+l and rows r0+16w .. r0+16w+15 (16 accumulators).
+// Blocks without a valid cell only copy the mask's nodata values.
 // The workgroup walks the candidate bins, classifies every point of those
 // bins against the block exactly (EMPTY: no cell inside the cut-off; FULL:
 // every cell inside; PARTIAL), queues the non-empty ones in bin order, and
 // consumes them in sub-batches of DT_SB points:
-//   1. a table Ey[p][row] = coef_p * exp2(dy2 * a_p) (and dy2 for PARTIAL
-//      points) is built cooperatively in LDS;
-//   2. each lane computes Ex = exp2(dx2 * a_p) once per point and adds
-//      Ex * Ey[p][row] to its 16 cells with one FMA each (FULL), or with the
-//      exact cut-off test fl64(dx2 + dy2) < thr first (PARTIAL);
+//   1. cooperatively, in LDS: Ex[p][col] = exp2(dx2 a_p) for the 64 columns,
+//      Ey[p][row] = coef_p exp2(dy2 a_p) and dy2 for the 64 rows, and for
+//      PARTIAL points the class of each wave's 64x16 sub-block (the same exact
+//      EMPTY / FULL test on the smaller rectangle);
+//   2. each wave adds Ex * Ey[p][row] to its 16 cells with one FMA each
+//      (FULL), or with the exact cut-off test fl64(dx2 + dy2) < thr first
+//      (PARTIAL), and skips EMPTY sub-blocks;
 //   3. the FP32 partial sums are Kahan-folded after every sub-batch.
 // The order of points is fixed (bins in row-major order, points in bin order)
 // so results are deterministic.
@@ -247,13 +250,20 @@ __global__ void k_bin_stats(int nbins, const int *bin_start, const PtRec *pts, B
 #define DT_DYP 66   // doubles per point row of the dy2 table (64 + pad)
 #define DT_Q (DT_SB + 256)
 
+// stats[0] candidate points classified, [1] FULL blocks, [2] PARTIAL blocks,
+// [3] FULL / [4] PARTIAL / [5] EMPTY wave sub-blocks of PARTIAL points,
+// [6] blocks skipped (no valid cell)
+#define DT_NSTATS 8
+
 struct TileArgs {
     GridDesc g;
     const float *mask;   // tile values (nullable: all valid)
     float *out;
     PointIndex idx;
+    unsigned long long *stats;   // nullable
 };
 
+template <bool STATS>
 __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
 {
     __shared__ int s_bins[256];
@@ -264,7 +274,8 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
     __shared__ float s_coef[DT_SB];
     __shared__ double s_a[DT_SB];
     __shared__ double s_thr[DT_SB];
-    __shared__ int s_part[DT_SB];
+    __shared__ int s_wcls[DT_SB];   // 2 bits per wave: 0 skip, 1 full, 2 partial
+    __shared__ __attribute__((aligned(16))) float s_ex[DT_SB * 64];
     __shared__ __attribute__((aligned(16))) float s_ey[DT_SB * DT_EYP];
     __shared__ __attribute__((aligned(16))) double s_dy2[DT_SB * DT_DYP];
 
@@ -274,15 +285,42 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
     const int64_t c0 = (int64_t)blockIdx.x * 64, r0 = (int64_t)blockIdx.y * 64;
     const int64_t c1 = (c0 + 63 < g.ncols - 1) ? c0 + 63 : g.ncols - 1;
     const int64_t r1 = (r0 + 63 < g.nrows - 1) ? r0 + 63 : g.nrows - 1;
+    const int64_t col = c0 + lane;
+
+    // validity of this thread's cells; blocks without a valid cell are skipped
+    unsigned vbits = 0;
+    if (col <= c1) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int64_t row = r0 + wave * 16 + j;
+            if (row <= r1) {
+                bool v = true;
+                if (A.mask) v = A.mask[row * g.ncols + col] != g.nodata;
+                vbits |= (v ? 1u : 0u) << j;
+            }
+        }
+    }
+    if (!__syncthreads_or(vbits != 0)) {
+        if (col <= c1) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int64_t row = r0 + wave * 16 + j;
+                if (row <= r1) A.out[row * g.ncols + col] = A.mask[row * g.ncols + col];
+            }
+        }
+        if (STATS && tid == 0) atomicAdd(&A.stats[6], 1ull);
+        return;
+    }
 
     // Block extent in cell-centre coordinates (exact N2 floats).
     const float bx0 = cell_x(xll, cs, c0), bx1 = cell_x(xll, cs, c1);
     const float by1 = cell_y(yll, cs, nrm05, r0), by0 = cell_y(yll, cs, nrm05, r1);
-    const float cx = cell_x(xll, cs, c0 + lane);
+    const float cx = cell_x(xll, cs, col);
 
     float acc[16], sum[16], comp[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) { acc[j] = 0.0f; sum[j] = 0.0f; comp[j] = 0.0f; }
+    unsigned st_cand = 0, st_full = 0, st_part = 0, st_wf = 0, st_wp = 0, st_we = 0;
 
     const PointIndex &I = A.idx;
     int bxa, bxb, bya, byb;
@@ -290,10 +328,10 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
     const int nbx_r = bxb - bxa + 1;
     const int nb_total = nbx_r * (byb - bya + 1);
 
-    int qn = 0;   // queue length (uniform, mirrored in registers)
+    int qn = 0;   // queue length (uniform)
 
     auto process = [&](int cnt) {
-        // 1. point parameters
+        // 1a. point parameters
         if (tid < cnt) {
             unsigned q = s_queue[tid];
             const PtRec p = I.pts[q & 0x7fffffffu];
@@ -302,42 +340,62 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
             s_coef[tid] = p.coef;
             s_a[tid] = p.a;
             s_thr[tid] = p.thr;
-            s_part[tid] = (int)(q >> 31);
+            // wave sub-block classes
+            int cl4 = 0x55;   // 2 bits per wave: 0 skip, 1 full, 2 partial
+            if (q >> 31) {
+                cl4 = 0;
+                const float fx = far1(p.x, bx0, bx1), nx = gap1(p.x, bx0, bx1);
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    const int64_t wr0 = r0 + 16 * w;
+                    const int64_t wr1 = (wr0 + 15 < r1) ? wr0 + 15 : r1;
+                    int cw = 0;
+                    if (wr0 <= r1) {
+                        const float wy1 = cell_y(yll, cs, nrm05, wr0), wy0 = cell_y(yll, cs, nrm05, wr1);
+                        if (sq2(nx, gap1(p.y, wy0, wy1)) < p.thr) cw = sq2(fx, far1(p.y, wy0, wy1)) < p.thr ? 1 : 2;
+                    }
+                    cl4 |= cw << (2 * w);
+                }
+            }
+            s_wcls[tid] = cl4;
         }
         __syncthreads();
-        // 2. Ey / dy2 tables: thread -> point (tid & 31), rows 8*(tid>>5) .. +7
+        // 1b. tables: thread -> point (tid & 31), rows / columns 8*(tid>>5) .. +7
         {
             const int p = tid & (DT_SB - 1), rg = tid >> 5;
             if (p < cnt) {
-                const float py = s_py[p], coef = s_coef[p];
+                const float py = s_py[p], px = s_px[p], coef = s_coef[p];
                 const double a = s_a[p];
 #pragma unroll
                 for (int k = 0; k < 8; k += 4) {
-                    float4 e4;
+                    float4 e4, x4;
                     double d2v[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const int row = rg * 8 + k + u;
-                        float cy = cell_y(yll, cs, nrm05, r0 + row);
-                        double dy2 = sqd(cy - py);
+                        const float cy = cell_y(yll, cs, nrm05, r0 + row);
+                        const double dy2 = sqd(cy - py);
                         d2v[u] = dy2;
-                        float e = fast_exp2((float)(dy2 * a)) * coef;
-                        (&e4.x)[u] = e;
+                        (&e4.x)[u] = fast_exp2((float)(dy2 * a)) * coef;
+                        const float cxx = cell_x(xll, cs, c0 + row);   // column index c0 + (same 0..63 slot)
+                        (&x4.x)[u] = fast_exp2((float)(sqd(cxx - px) * a));
                     }
                     *(float4 *)&s_ey[p * DT_EYP + rg * 8 + k] = e4;
+                    *(float4 *)&s_ex[p * 64 + rg * 8 + k] = x4;
                     *(double2 *)&s_dy2[p * DT_DYP + rg * 8 + k] = make_double2(d2v[0], d2v[1]);
                     *(double2 *)&s_dy2[p * DT_DYP + rg * 8 + k + 2] = make_double2(d2v[2], d2v[3]);
                 }
             }
         }
         __syncthreads();
-        // 3. accumulate
+        // 2. accumulate
         for (int p = 0; p < cnt; ++p) {
-            const float px = s_px[p];
-            const double a = s_a[p];
-            const float dx = cx - px;
-            const double dx2 = sqd(dx);
-            const float ex = fast_exp2((float)(dx2 * a));
+            const int cls = (__builtin_amdgcn_readfirstlane(s_wcls[p]) >> (2 * wave)) & 3;
+            if (cls == 0) {
+                if (STATS && lane == 0) ++st_we;
+                continue;
+            }
+            const float ex = s_ex[p * 64 + lane];
             const float4 *eyp = (const float4 *)&s_ey[p * DT_EYP + wave * 16];
             float ey[16];
 #pragma unroll
@@ -348,12 +406,13 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
                 ey[4 * k + 2] = v.z;
                 ey[4 * k + 3] = v.w;
             }
-            const int part = __builtin_amdgcn_readfirstlane(s_part[p]);
-            if (!part) {
+            if (cls == 1) {
 #pragma unroll
                 for (int j = 0; j < 16; ++j) acc[j] = __builtin_fmaf(ex, ey[j], acc[j]);
+                if (STATS && lane == 0) ++st_wf;
             } else {
                 const double thr = s_thr[p];
+                const double dx2 = sqd(cx - s_px[p]);
                 const double2 *dyp = (const double2 *)&s_dy2[p * DT_DYP + wave * 16];
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
@@ -363,6 +422,7 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
                     acc[2 * k] = __builtin_fmaf(in0 ? ex : 0.0f, ey[2 * k], acc[2 * k]);
                     acc[2 * k + 1] = __builtin_fmaf(in1 ? ex : 0.0f, ey[2 * k + 1], acc[2 * k + 1]);
                 }
+                if (STATS && lane == 0) ++st_wp;
             }
         }
 #pragma unroll
@@ -397,6 +457,7 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
                 int cls = 0;   // 0 empty, 1 full, 2 partial
                 if (i < en) {
                     const PtRec p = I.pts[i];
+                    if (STATS) ++st_cand;
                     if (p.thr > 0.0) {
                         double near2 = sq2(gap1(p.x, bx0, bx1), gap1(p.y, by0, by1));
                         if (near2 < p.thr) {
@@ -404,6 +465,8 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
                             cls = far2 < p.thr ? 1 : 2;
                         }
                     }
+                    if (STATS && cls == 1) ++st_full;
+                    if (STATS && cls == 2) ++st_part;
                 }
                 int na;
                 int pos = block_compact256(cls != 0, s_wcnt, &na);
@@ -440,21 +503,28 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
     if (qn > 0) process(qn);
 
     // write back: nodata cells keep the mask value
-    const int64_t col = c0 + lane;
     if (col <= c1) {
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             const int64_t row = r0 + wave * 16 + j;
             if (row <= r1) {
                 const int64_t ix = row * g.ncols + col;
-                float v = sum[j];
-                if (A.mask) {
-                    float m = A.mask[ix];
-                    if (m == g.nodata) v = m;
-                }
-                A.out[ix] = v;
+                A.out[ix] = ((vbits >> j) & 1u) ? sum[j] : A.mask[ix];
             }
         }
+    }
+    if (STATS) {
+        auto put = [&](int slot, unsigned v) {
+            unsigned long long t = v;
+            for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d, WAVE);
+            if (lane == 0 && t) atomicAdd(&A.stats[slot], t);
+        };
+        put(0, st_cand);
+        put(1, st_full);
+        put(2, st_part);
+        put(3, st_wf);
+        put(4, st_wp);
+        put(5, st_we);
     }
 }
 
@@ -1209,11 +1279,13 @@ hipError_t bin_stats(hipStream_t s, int nbins, const int *bin_start, const PtRec
     return hipGetLastError();
 }
 
-hipError_t density_tile(hipStream_t s, const GridDesc &g, const float *mask, float *out, const PointIndex &idx)
+hipError_t density_tile(hipStream_t s, const GridDesc &g, const float *mask, float *out, const PointIndex &idx,
+                        unsigned long long *stats)
 {
-    TileArgs A{g, mask, out, idx};
+    TileArgs A{g, mask, out, idx, stats};
     dim3 grid((unsigned)((g.ncols + 63) / 64), (unsigned)((g.nrows + 63) / 64));
-    hipLaunchKernelGGL(k_density_tile, grid, dim3(256), 0, s, A);
+    if (stats) hipLaunchKernelGGL(k_density_tile<true>, grid, dim3(256), 0, s, A);
+    else hipLaunchKernelGGL(k_density_tile<false>, grid, dim3(256), 0, s, A);
     return hipGetLastError();
 }
 
