@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <numeric>
@@ -26,7 +27,8 @@ hipError_t prep_points(hipStream_t, int64_t, const int *, const float *, const f
                        const float *, float, PtRec *);
 hipError_t bin_stats(hipStream_t, int, const int *, const PtRec *, BinInfo *, double *);
 hipError_t density_tile(hipStream_t, const GridDesc &, const float *, float *, const PointIndex &,
-                        unsigned long long *);
+                        unsigned long long *, const int *);
+hipError_t block_order(hipStream_t, const GridDesc &, const float *, const PointIndex &, unsigned *, unsigned *, int *);
 hipError_t point_density(hipStream_t, const PointIndex &, int, float *);
 hipError_t loo_multi(hipStream_t, const PointIndex &, const float *, const double *, const double *, int, float *);
 hipError_t range_count(hipStream_t, const PointIndex &, int64_t *);
@@ -171,6 +173,10 @@ struct mkde_ctx {
     // density kernel counters (mkde_density_stats)
     bool dt_stats_on = false;
     DevBuf<unsigned long long> dt_stats;
+    // heaviest-first block order of the density kernel
+    bool dt_order_on = true;
+    DevBuf<unsigned> blk_cost, blk_hist;
+    DevBuf<int> blk_order;
 };
 
 extern "C" const char *mkde_last_error(void) { return g_err.c_str(); }
@@ -194,6 +200,9 @@ extern "C" int mkde_ctx_create(int device, mkde_ctx **out)
     mkde_ctx *c = new (std::nothrow) mkde_ctx();
     if (!c) return fail(MKDE_E_NOMEM, "out of host memory");
     c->device = device;
+    // tuning knob: MKDE_DENSITY_ORDER=0 visits density blocks in raster order
+    const char *ord = getenv("MKDE_DENSITY_ORDER");
+    if (ord && ord[0] == '0') c->dt_order_on = false;
     e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
@@ -978,9 +987,19 @@ extern "C" int mkde_density_tile_resident(mkde_ctx *c)
     if (!c->tile_set) return fail(MKDE_E_STATE, "no tile uploaded");
     CHK(prep(c));
     PointIndex I = index_of(c);
+    const float *tm = c->tile_has_mask ? c->tmask.p : nullptr;
+    const int *order = nullptr;
+    if (c->dt_order_on) {
+        const int64_t nblk = ((c->tg.ncols + 63) / 64) * ((c->tg.nrows + 63) / 64);
+        CHK(c->blk_cost.ensure(nblk));
+        CHK(c->blk_hist.ensure(32));
+        CHK(c->blk_order.ensure(nblk));
+        HIPCHK(mkde_dev::block_order(c->stream, c->tg, tm, I, c->blk_cost.p, c->blk_hist.p, c->blk_order.p));
+        order = c->blk_order.p;
+    }
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(mkde_dev::density_tile(c->stream, c->tg, c->tile_has_mask ? c->tmask.p : nullptr, c->tout.p, I,
-                                  c->dt_stats_on ? c->dt_stats.p : nullptr));
+    HIPCHK(mkde_dev::density_tile(c->stream, c->tg, tm, c->tout.p, I, c->dt_stats_on ? c->dt_stats.p : nullptr,
+                                  order));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     return MKDE_OK;
 }

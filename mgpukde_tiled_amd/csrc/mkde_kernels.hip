@@ -261,6 +261,8 @@ struct TileArgs {
     float *out;
     PointIndex idx;
     unsigned long long *stats;   // nullable
+    const int *order;            // nullable: block visiting order (heaviest first)
+    int nbx;                     // blocks per row of the tile
 };
 
 template <bool STATS>
@@ -275,6 +277,7 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
     __shared__ double s_a[DT_SB];
     __shared__ double s_thr[DT_SB];
     __shared__ int s_wcls[DT_SB];   // 2 bits per wave: 0 skip, 1 full, 2 partial
+    __shared__ int s_nfull;         // block-FULL points at the front of the sub-batch
     __shared__ __attribute__((aligned(16))) float s_ex[DT_SB * 64];
     __shared__ __attribute__((aligned(16))) float s_ey[DT_SB * DT_EYP];
     __shared__ __attribute__((aligned(16))) double s_dy2[DT_SB * DT_DYP];
@@ -282,7 +285,8 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const GridDesc g = A.g;
     const double xll = g.xll, yll = g.yll, cs = g.cs, nrm05 = (double)g.nrows - 0.5;
-    const int64_t c0 = (int64_t)blockIdx.x * 64, r0 = (int64_t)blockIdx.y * 64;
+    const int blk = A.order ? A.order[blockIdx.x] : (int)blockIdx.x;
+    const int64_t c0 = (int64_t)(blk % A.nbx) * 64, r0 = (int64_t)(blk / A.nbx) * 64;
     const int64_t c1 = (c0 + 63 < g.ncols - 1) ? c0 + 63 : g.ncols - 1;
     const int64_t r1 = (r0 + 63 < g.nrows - 1) ? r0 + 63 : g.nrows - 1;
     const int64_t col = c0 + lane;
@@ -331,15 +335,24 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
     int qn = 0;   // queue length (uniform)
 
     auto process = [&](int cnt) {
-        // 1a. point parameters
-        if (tid < cnt) {
-            unsigned q = s_queue[tid];
+        // 1a. point parameters, block-FULL points first (stable partition, so
+        //     the order stays deterministic); cnt <= DT_SB <= 64: one wave
+        if (tid < 64) {
+            const unsigned q = tid < cnt ? s_queue[tid] : 0u;
+            const bool isfull = tid < cnt && !(q >> 31);
+            const uint64_t fm = __ballot(isfull);
+            const uint64_t below = (1ull << tid) - 1ull;
+            const int nf = __popcll(fm);
+            const int slot = isfull ? __popcll(fm & below) : nf + (tid - __popcll(fm & below));
+            if (tid == 0) s_nfull = nf;
+            if (tid < cnt) {
             const PtRec p = I.pts[q & 0x7fffffffu];
-            s_px[tid] = p.x;
-            s_py[tid] = p.y;
-            s_coef[tid] = p.coef;
-            s_a[tid] = p.a;
-            s_thr[tid] = p.thr;
+            const int t = slot;
+            s_px[t] = p.x;
+            s_py[t] = p.y;
+            s_coef[t] = p.coef;
+            s_a[t] = p.a;
+            s_thr[t] = p.thr;
             // wave sub-block classes
             int cl4 = 0x55;   // 2 bits per wave: 0 skip, 1 full, 2 partial
             if (q >> 31) {
@@ -357,7 +370,8 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
                     cl4 |= cw << (2 * w);
                 }
             }
-            s_wcls[tid] = cl4;
+            s_wcls[t] = cl4;
+            }
         }
         __syncthreads();
         // 1b. tables: thread -> point (tid & 31), rows / columns 8*(tid>>5) .. +7
@@ -388,8 +402,24 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
             }
         }
         __syncthreads();
-        // 2. accumulate
-        for (int p = 0; p < cnt; ++p) {
+        // 2a. block-FULL points: one FMA per cell, no tests
+        const int nfull = s_nfull;
+#pragma unroll 2
+        for (int p = 0; p < nfull; ++p) {
+            const float ex = s_ex[p * 64 + lane];
+            const float4 *eyp = (const float4 *)&s_ey[p * DT_EYP + wave * 16];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float4 v = eyp[k];
+                acc[4 * k] = __builtin_fmaf(ex, v.x, acc[4 * k]);
+                acc[4 * k + 1] = __builtin_fmaf(ex, v.y, acc[4 * k + 1]);
+                acc[4 * k + 2] = __builtin_fmaf(ex, v.z, acc[4 * k + 2]);
+                acc[4 * k + 3] = __builtin_fmaf(ex, v.w, acc[4 * k + 3]);
+            }
+            if (STATS && lane == 0) ++st_wf;
+        }
+        // 2b. block-PARTIAL points: per-wave class
+        for (int p = nfull; p < cnt; ++p) {
             const int cls = (__builtin_amdgcn_readfirstlane(s_wcls[p]) >> (2 * wave)) & 3;
             if (cls == 0) {
                 if (STATS && lane == 0) ++st_we;
@@ -526,6 +556,81 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
         put(4, st_wp);
         put(5, st_we);
     }
+}
+
+// Work estimate of each 64x64 block: points in the candidate bins (0 for
+// blocks without a valid cell), then a heaviest-first visiting order by
+// log2 buckets, so the long blocks start early and the launch drains evenly.
+__global__ __launch_bounds__(256) void k_block_cost(GridDesc g, const float *mask, PointIndex I, int nbx,
+                                                    unsigned *cost)
+{
+    __shared__ unsigned s_sum[4];
+    __shared__ int s_any;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int blk = blockIdx.x;
+    const double xll = g.xll, yll = g.yll, cs = g.cs, nrm05 = (double)g.nrows - 0.5;
+    const int64_t c0 = (int64_t)(blk % nbx) * 64, r0 = (int64_t)(blk / nbx) * 64;
+    const int64_t c1 = (c0 + 63 < g.ncols - 1) ? c0 + 63 : g.ncols - 1;
+    const int64_t r1 = (r0 + 63 < g.nrows - 1) ? r0 + 63 : g.nrows - 1;
+    bool v = false;
+    if (mask) {
+        for (int k = tid; k < 64 * 64; k += 256) {
+            const int64_t r = r0 + k / 64, c = c0 + k % 64;
+            if (r <= r1 && c <= c1 && mask[r * g.ncols + c] != g.nodata) v = true;
+        }
+    } else {
+        v = true;
+    }
+    const int any = __syncthreads_or(v);
+    const float bx0 = cell_x(xll, cs, c0), bx1 = cell_x(xll, cs, c1);
+    const float by1 = cell_y(yll, cs, nrm05, r0), by0 = cell_y(yll, cs, nrm05, r1);
+    unsigned acc = 0;
+    if (any) {
+        int bxa, bxb, bya, byb;
+        bin_range(I.bg, bx0, bx1, by0, by1, *I.reach_p, bxa, bxb, bya, byb);
+        const int nbx_r = bxb - bxa + 1;
+        const int nb_total = nbx_r * (byb - bya + 1);
+        for (int k = tid; k < nb_total; k += 256) {
+            const int b = (bya + k / nbx_r) * I.bg.nbx + (bxa + k % nbx_r);
+            const int cntb = I.bin_start[b + 1] - I.bin_start[b];
+            if (cntb > 0 && box_gap2(I.bin_box[b], bx0, bx1, by0, by1) < I.bin_thr[b]) acc += (unsigned)cntb;
+        }
+    }
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, WAVE);
+    if (lane == 0) s_sum[wave] = acc;
+    __syncthreads();
+    if (tid == 0) cost[blk] = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
+    (void)s_any;
+}
+
+__device__ __forceinline__ int cost_bucket(unsigned c)
+{
+    // 32 buckets, heaviest first: bucket 0 holds the largest costs
+    return c == 0 ? 31 : (__clz(c) < 31 ? __clz(c) : 30);
+}
+
+__global__ void k_bucket_hist(int nblk, const unsigned *cost, unsigned *hist)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nblk) atomicAdd(&hist[cost_bucket(cost[b])], 1u);
+}
+
+__global__ void k_bucket_scan(unsigned *hist)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        unsigned run = 0;
+        for (int k = 0; k < 32; ++k) {
+            unsigned c = hist[k];
+            hist[k] = run;
+            run += c;
+        }
+    }
+}
+
+__global__ void k_bucket_fill(int nblk, const unsigned *cost, unsigned *offs, int *order)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nblk) order[atomicAdd(&offs[cost_bucket(cost[b])], 1u)] = b;
 }
 
 // ==================================================== density at points ====
@@ -1264,6 +1369,7 @@ extern "C++" {
 namespace mkde_dev {
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+static inline unsigned nblk_(int64_t n) { return nblk(n, 256); }
 
 hipError_t prep_points(hipStream_t s, int64_t n, const int *perm, const float *xs, const float *ys,
                        const float *w, const float *h, const float *ew, float h_fixed, PtRec *out)
@@ -1280,12 +1386,26 @@ hipError_t bin_stats(hipStream_t s, int nbins, const int *bin_start, const PtRec
 }
 
 hipError_t density_tile(hipStream_t s, const GridDesc &g, const float *mask, float *out, const PointIndex &idx,
-                        unsigned long long *stats)
+                        unsigned long long *stats, const int *order)
 {
-    TileArgs A{g, mask, out, idx, stats};
-    dim3 grid((unsigned)((g.ncols + 63) / 64), (unsigned)((g.nrows + 63) / 64));
+    const int nbx = (int)((g.ncols + 63) / 64), nby = (int)((g.nrows + 63) / 64);
+    TileArgs A{g, mask, out, idx, stats, order, nbx};
+    dim3 grid((unsigned)(nbx * nby));
     if (stats) hipLaunchKernelGGL(k_density_tile<true>, grid, dim3(256), 0, s, A);
     else hipLaunchKernelGGL(k_density_tile<false>, grid, dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t block_order(hipStream_t s, const GridDesc &g, const float *mask, const PointIndex &idx, unsigned *cost,
+                       unsigned *hist, int *order)
+{
+    const int nbx = (int)((g.ncols + 63) / 64), nby = (int)((g.nrows + 63) / 64);
+    const int nblk = nbx * nby;
+    hipLaunchKernelGGL(k_block_cost, dim3((unsigned)nblk), dim3(256), 0, s, g, mask, idx, nbx, cost);
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, (int64_t)32, 0.0f, (float *)hist);
+    hipLaunchKernelGGL(k_bucket_hist, dim3(nblk_(nblk)), dim3(256), 0, s, nblk, cost, hist);
+    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(64), 0, s, hist);
+    hipLaunchKernelGGL(k_bucket_fill, dim3(nblk_(nblk)), dim3(256), 0, s, nblk, cost, hist, order);
     return hipGetLastError();
 }
 
