@@ -29,7 +29,7 @@ hipError_t bin_stats(hipStream_t, int, const int *, const PtRec *, BinInfo *, do
 hipError_t density_tile(hipStream_t, const GridDesc &, const float *, float *, const PointIndex &,
                         unsigned long long *, const int *);
 hipError_t block_order(hipStream_t, const GridDesc &, const float *, const PointIndex &, unsigned *, unsigned *, int *);
-hipError_t point_density(hipStream_t, const PointIndex &, int, float *);
+hipError_t point_density(hipStream_t, const PointIndex &, int, float *, float);
 hipError_t loo_multi(hipStream_t, const PointIndex &, const float *, const double *, const double *, int, float *);
 hipError_t range_count(hipStream_t, const PointIndex &, int64_t *);
 hipError_t range_fill(hipStream_t, const PointIndex &, const int64_t *, int32_t *);
@@ -704,7 +704,7 @@ static int density_points_sorted(mkde_ctx *c, int loo, float *den_sorted)
     CHK(prep(c));
     PointIndex I = index_of(c);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(mkde_dev::point_density(c->stream, I, loo, den_sorted));
+    HIPCHK(mkde_dev::point_density(c->stream, I, loo, den_sorted, c->h_fixed));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     return MKDE_OK;
 }

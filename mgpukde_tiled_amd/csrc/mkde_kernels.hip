@@ -636,36 +636,54 @@ __global__ void k_bucket_fill(int nblk, const unsigned *cost, unsigned *offs, in
 // ==================================================== density at points ====
 //
 // One 256-thread workgroup owns 256 consecutive query points in bin order.
-// Source points of the candidate bins are staged through LDS 256 at a time;
-// every thread tests and accumulates all of them for its query.  Each source
-// uses its own bandwidth (sample-point estimator).  LOO excludes the query
-// itself (same index); duplicates at distance 0 are kept.
+// Source points of the candidate bins are streamed through LDS 256 at a time
+// (several bins per batch), and every thread tests and accumulates all of
+// them for its query.  Each source uses its own bandwidth (sample-point
+// estimator); FIXED specialises the common case of one bandwidth.  LOO
+// excludes the query itself (same index); duplicates at distance 0 are kept.
+//
+// Cut-off test: d2f = fl32(dx*dx + dy*dy) decides unless it lies within a
+// relative 2^-20 band around the threshold (its error is below 2^-22
+// relative); for a wave with any lane in the band the exact FP64 test
+// fl64(dx^2 + dy^2) < thr is evaluated instead, so the result is exact.
 
 struct PtsArgs {
     PointIndex idx;
     float *den;   // output, bin order
     int loo;
+    double thr_fixed, a_fixed;   // FIXED kernels only
 };
 
-__global__ __launch_bounds__(256) void k_point_density(PtsArgs A)
-{
-    __shared__ int s_bins[256];
-    __shared__ int s_wcnt[4];
-    __shared__ PtRec s_src[256];
-    __shared__ float s_red[4][4];
+#define PD_BAND (1.0 / 1048576.0)
 
-    const PointIndex &I = A.idx;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t q = (int64_t)blockIdx.x * 256 + tid;
-    const bool qv = q < I.n;
-    float qx = 0.0f, qy = 0.0f;
-    if (qv) {
-        PtRec p = I.pts[q];
-        qx = p.x;
-        qy = p.y;
+// Exclusive scan of one int per thread across a 256-thread block.
+__device__ __forceinline__ int block_exscan256(int v, int *s_w, int *total)
+{
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+    int inc = v;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        int o = __shfl_up(inc, d, WAVE);
+        if (lane >= d) inc += o;
     }
-    // bounding box of the group's queries
-    float mnx = qv ? qx : FLT_MAX, mxx = qv ? qx : -FLT_MAX, mny = qv ? qy : FLT_MAX, mxy = qv ? qy : -FLT_MAX;
+    if (lane == WAVE - 1) s_w[wave] = inc;
+    __syncthreads();
+    int off = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) off += (w < wave) ? s_w[w] : 0;
+    *total = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    __syncthreads();
+    return off + inc - v;
+}
+
+// Bounding box of the valid queries of the block (all threads get it).
+__device__ __forceinline__ void block_bbox(bool qv, float qx, float qy, float (*s_red)[4], float &mnx, float &mxx,
+                                           float &mny, float &mxy)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    mnx = qv ? qx : FLT_MAX;
+    mxx = qv ? qx : -FLT_MAX;
+    mny = qv ? qy : FLT_MAX;
+    mxy = qv ? qy : -FLT_MAX;
     for (int d = 32; d >= 1; d >>= 1) {
         mnx = fminf(mnx, __shfl_xor(mnx, d, WAVE));
         mxx = fmaxf(mxx, __shfl_xor(mxx, d, WAVE));
@@ -683,52 +701,138 @@ __global__ __launch_bounds__(256) void k_point_density(PtsArgs A)
     mxx = fmaxf(fmaxf(s_red[0][1], s_red[1][1]), fmaxf(s_red[2][1], s_red[3][1]));
     mny = fminf(fminf(s_red[0][2], s_red[1][2]), fminf(s_red[2][2], s_red[3][2]));
     mxy = fmaxf(fmaxf(s_red[0][3], s_red[1][3]), fmaxf(s_red[2][3], s_red[3][3]));
+    __syncthreads();
+}
 
+template <bool FIXED, bool LOO>
+__global__ __launch_bounds__(256) void k_point_density(PtsArgs A)
+{
+    __shared__ int s_bins[256];
+    __shared__ int s_pref[257];
+    __shared__ int s_w[4];
+    __shared__ float4 s_s4[256];      // x, y, coef, a (float)
+    __shared__ float2 s_band[256];    // thr*(1-band), thr*(1+band) as float
+    __shared__ double s_thr[256];
+    __shared__ int s_idx[256];
+    __shared__ float s_red[4][4];
+
+    const PointIndex &I = A.idx;
+    const int tid = threadIdx.x;
+    const int64_t q = (int64_t)blockIdx.x * 256 + tid;
+    const bool qv = q < I.n;
+    float qx = 0.0f, qy = 0.0f;
+    if (qv) {
+        const PtRec p = I.pts[q];
+        qx = p.x;
+        qy = p.y;
+    }
+    float mnx, mxx, mny, mxy;
+    block_bbox(qv, qx, qy, s_red, mnx, mxx, mny, mxy);
+
+    const double reach = FIXED ? sqrt(fmax(A.thr_fixed, 0.0)) * 1.000001 + 1e-6 : *I.reach_p;
     int bxa, bxb, bya, byb;
-    bin_range(I.bg, mnx, mxx, mny, mxy, *I.reach_p, bxa, bxb, bya, byb);
+    bin_range(I.bg, mnx, mxx, mny, mxy, reach, bxa, bxb, bya, byb);
     const int nbx_r = bxb - bxa + 1;
     const int nb_total = nbx_r * (byb - bya + 1);
 
+    const float f_thr_lo = FIXED ? (float)(A.thr_fixed * (1.0 - PD_BAND)) : 0.0f;
+    const float f_thr_hi = FIXED ? fmaxf((float)(A.thr_fixed * (1.0 + PD_BAND)), FLT_MIN) : 0.0f;
+    const float f_a = FIXED ? (float)A.a_fixed : 0.0f;
+
     float acc = 0.0f, sum = 0.0f, comp = 0.0f;
-    int nacc = 0;
     for (int base = 0; base < nb_total; base += 256) {
-        const int k = base + tid;
-        bool cand = false;
+        const int kk = base + tid;
+        int cntb = 0;
         int b = 0;
-        if (k < nb_total) {
-            b = (bya + k / nbx_r) * I.bg.nbx + (bxa + k % nbx_r);
-            if (I.bin_start[b + 1] > I.bin_start[b]) cand = box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < I.bin_thr[b];
-        }
-        int nb;
-        int slot = block_compact256(cand, s_wcnt, &nb);
-        if (cand) s_bins[slot] = b;
-        __syncthreads();
-        for (int kb = 0; kb < nb; ++kb) {
-            const int bb = s_bins[kb];
-            const int st = I.bin_start[bb], en = I.bin_start[bb + 1];
-            for (int off = st; off < en; off += 256) {
-                const int cnt = (en - off) < 256 ? (en - off) : 256;
-                if (tid < cnt) s_src[tid] = I.pts[off + tid];
-                __syncthreads();
-                for (int s = 0; s < cnt; ++s) {
-                    const PtRec p = s_src[s];
-                    const double d2 = sq2(p.x - qx, p.y - qy);
-                    bool in = d2 < p.thr;
-                    if (A.loo && (int64_t)(off + s) == q) in = false;
-                    const float e = fast_exp2((float)(d2 * p.a));
-                    acc = __builtin_fmaf(in ? e : 0.0f, p.coef, acc);
-                    if (++nacc == 32) {
-                        kahan_fold(acc, sum, comp);
-                        acc = 0.0f;
-                        nacc = 0;
-                    }
-                }
-                __syncthreads();
+        if (kk < nb_total) {
+            b = (bya + kk / nbx_r) * I.bg.nbx + (bxa + kk % nbx_r);
+            const int c = I.bin_start[b + 1] - I.bin_start[b];
+            if (c > 0) {
+                const double lim = FIXED ? A.thr_fixed : I.bin_thr[b];
+                if (box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < lim) cntb = c;
             }
         }
+        int nb;
+        const int slot = block_compact256(cntb > 0, s_w, &nb);
+        if (cntb > 0) s_bins[slot] = b;
+        int T;
+        // counts of the compacted bins, in slot order
         __syncthreads();
+        const int mycnt = tid < nb ? I.bin_start[s_bins[tid] + 1] - I.bin_start[s_bins[tid]] : 0;
+        const int pre = block_exscan256(mycnt, s_w, &T);
+        if (tid < nb) s_pref[tid] = pre;
+        if (tid == 0) s_pref[nb] = T;
+        __syncthreads();
+        for (int g0 = 0; g0 < T; g0 += 256) {
+            const int gidx = g0 + tid;
+            const int cnt = (T - g0) < 256 ? (T - g0) : 256;
+            if (gidx < T) {
+                int lo = 0, hi = nb - 1;   // last k with s_pref[k] <= gidx
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pref[mid] <= gidx) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int si = I.bin_start[s_bins[lo]] + (gidx - s_pref[lo]);
+                const PtRec p = I.pts[si];
+                if (FIXED) {
+                    s_s4[tid] = make_float4(p.x, p.y, p.coef, 0.0f);
+                } else {
+                    s_s4[tid] = make_float4(p.x, p.y, p.coef, (float)p.a);
+                    s_band[tid] = make_float2((float)(p.thr * (1.0 - PD_BAND)),
+                                              fmaxf((float)(p.thr * (1.0 + PD_BAND)), FLT_MIN));
+                    s_thr[tid] = p.thr;
+                }
+                s_idx[tid] = si;
+            }
+            __syncthreads();
+            for (int s0 = 0; s0 < cnt; s0 += 32) {
+                const int s1 = (s0 + 32 < cnt) ? s0 + 32 : cnt;
+                // fast pass: FP32 cut-off, noting sources inside the guard band
+                bool band = false;
+#pragma unroll 4
+                for (int s = s0; s < s1; ++s) {
+                    const float4 src = s_s4[s];
+                    const float dx = src.x - qx, dy = src.y - qy;
+                    const float d2f = __builtin_fmaf(dy, dy, dx * dx);
+                    float tlo, thi, af;
+                    if (FIXED) {
+                        tlo = f_thr_lo;
+                        thi = f_thr_hi;
+                        af = f_a;
+                    } else {
+                        const float2 bd = s_band[s];
+                        tlo = bd.x;
+                        thi = bd.y;
+                        af = src.w;
+                    }
+                    bool in = d2f < tlo;
+                    band |= (d2f < thi) != in;
+                    if (LOO && (int64_t)s_idx[s] == q) in = false;
+                    const float e = fast_exp2(d2f * af);
+                    acc = __builtin_fmaf(in ? e : 0.0f, src.z, acc);
+                }
+                // rare: some lane of the wave met a source in the band -- redo
+                // the chunk with the exact FP64 test for the whole wave
+                if (__any(band)) {
+                    acc = 0.0f;
+                    for (int s = s0; s < s1; ++s) {
+                        const float4 src = s_s4[s];
+                        const float dx = src.x - qx, dy = src.y - qy;
+                        const float d2f = __builtin_fmaf(dy, dy, dx * dx);
+                        const double t = FIXED ? A.thr_fixed : s_thr[s];
+                        bool in = sq2(dx, dy) < t;
+                        if (LOO && (int64_t)s_idx[s] == q) in = false;
+                        const float e = fast_exp2(d2f * (FIXED ? f_a : src.w));
+                        acc = __builtin_fmaf(in ? e : 0.0f, src.z, acc);
+                    }
+                }
+                kahan_fold(acc, sum, comp);
+                acc = 0.0f;
+            }
+            __syncthreads();
+        }
     }
-    kahan_fold(acc, sum, comp);
     if (qv) A.den[q] = sum;
 }
 
@@ -1003,6 +1107,9 @@ __global__ void k_boundary(GridDesc g, const uint8_t *valid, uint8_t *flags, flo
 
 // CalcDist2Boundary (CPU semantics, tiled:4091-4131): min over boundary cells
 // of fl64(dx^2 + dy^2), DBL_MAX if none.  Boundary centres streamed via LDS.
+// A float screen skips a cell when fl32(dx*dx + dy*dy) exceeds the running
+// minimum by more than a relative 2^-20 (its error is below 2^-22 relative),
+// so only cells that can lower the minimum take the exact FP64 path.
 __global__ __launch_bounds__(256) void k_dist_boundary(int64_t n, const float *x, const float *y, const float2 *blist,
                                                        int64_t nb, double *dist)
 {
@@ -1010,14 +1117,23 @@ __global__ __launch_bounds__(256) void k_dist_boundary(int64_t n, const float *x
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const float px = i < n ? x[i] : 0.0f, py = i < n ? y[i] : 0.0f;
     double m = DBL_MAX;
+    float mf = FLT_MAX;   // screen: cells with d2f > mf cannot lower m
     for (int64_t base = 0; base < nb; base += 1024) {
         const int cnt = (nb - base) < 1024 ? (int)(nb - base) : 1024;
         for (int t = threadIdx.x; t < cnt; t += blockDim.x) s_b[t] = blist[base + t];
         __syncthreads();
         for (int t = 0; t < cnt; ++t) {
             const float2 b = s_b[t];
-            const double d2 = sq2(b.x - px, b.y - py);
-            m = d2 < m ? d2 : m;
+            const float dx = b.x - px, dy = b.y - py;
+            const float d2f = __builtin_fmaf(dy, dy, dx * dx);
+            if (d2f <= mf) {
+                const double d2 = sq2(dx, dy);
+                if (d2 < m) {
+                    m = d2;
+                    mf = (float)(m * (1.0 + 1.0 / 1048576.0));
+                    if (!(mf < FLT_MAX)) mf = FLT_MAX;
+                }
+            }
         }
         __syncthreads();
     }
@@ -1409,10 +1525,24 @@ hipError_t block_order(hipStream_t s, const GridDesc &g, const float *mask, cons
     return hipGetLastError();
 }
 
-hipError_t point_density(hipStream_t s, const PointIndex &idx, int loo, float *den_sorted)
+hipError_t point_density(hipStream_t s, const PointIndex &idx, int loo, float *den_sorted, float h_fixed)
 {
-    PtsArgs A{idx, den_sorted, loo};
-    if (idx.n > 0) hipLaunchKernelGGL(k_point_density, dim3(nblk(idx.n, 256)), dim3(256), 0, s, A);
+    PtsArgs A{idx, den_sorted, loo, 0.0, 0.0};
+    if (idx.n <= 0) return hipSuccess;
+    if (h_fixed > 0.0f) {
+        const double h2 = (double)(float)(h_fixed * h_fixed);
+        A.thr_fixed = 9.0 * h2;
+        A.a_fixed = -MKDE_LOG2E / (2.0 * h2);
+        if (loo) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density<true, true>), dim3(nblk(idx.n, 256)), dim3(256), 0,
+                                    s, A);
+        else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density<true, false>), dim3(nblk(idx.n, 256)), dim3(256), 0,
+                                s, A);
+    } else {
+        if (loo) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density<false, true>), dim3(nblk(idx.n, 256)), dim3(256),
+                                    0, s, A);
+        else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density<false, false>), dim3(nblk(idx.n, 256)), dim3(256),
+                                0, s, A);
+    }
     return hipGetLastError();
 }
 

@@ -183,7 +183,10 @@ def test_density_at_points(ctx, orc, scene, adaptive):
     hs = r.uniform(5.0, 20.0, n).astype(np.float32) if adaptive else np.full(n, 9.0, np.float32)
     ew = r.uniform(1.0, 1.5, n).astype(np.float32)
     ctx.points_upload(s["x"], s["y"], s["w"])
-    ctx.set_bandwidths(hs)
+    if adaptive:
+        ctx.set_bandwidths(hs)
+    else:
+        ctx.set_bandwidth(9.0)  # the single-bandwidth kernel
     ctx.set_ec_weights(ew)
     d0 = ctx.DensityAtPoints(loo=False)
     d1 = ctx.DensityAtPoints(loo=True)
@@ -232,3 +235,28 @@ def test_abramson(ctx, orc, scene):
     assert abs(g - g64) <= 1e-6 * g64
     h64 = orc.abramson64(h0, -0.5, g64, p64)
     assert rel_err(hs, h64).max() <= REL
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_density_at_points_cutoff_ties(ctx, orc, adaptive):
+    """Integer coordinates with h = 1 (or h in {1, 2}): many pairs sit exactly
+    on the cut-off d2 = 9 h2 and must be excluded, as in the oracle."""
+    r = np.random.default_rng(17)
+    n = 2000
+    x = r.integers(0, 40, n).astype(np.float32)
+    y = r.integers(0, 40, n).astype(np.float32)
+    w = np.ones(n, np.float32)
+    hs = (r.integers(1, 3, n) if adaptive else np.ones(n)).astype(np.float32)
+    ctx.points_upload(x, y, w)
+    if adaptive:
+        ctx.set_bandwidths(hs)
+    else:
+        ctx.set_bandwidth(1.0)
+    ctx.set_ec_weights(None)
+    d0 = ctx.DensityAtPoints(loo=False)
+    d1 = ctx.DensityAtPoints(loo=True)
+    o = orc.loo_density(x, y, w, w, hs)
+    assert rel_err(d0, o["den0_64"]).max() <= REL
+    pos = o["den1_64"] > 0
+    assert np.all(d1[~pos] == 0)
+    assert rel_err(d1[pos], o["den1_64"][pos]).max() <= REL
