@@ -227,6 +227,47 @@ typedef struct {
 
 int mkde_run_job(mkde_ctx *ctx, const mkde_job_opts *opts, mkde_job_stats *stats);
 
+/* Free / total device memory of the context's GPU (the reference sizes its
+ * raster tiles from cudaMemGetInfo, tiled:1152-1165). */
+int mkde_device_memory(mkde_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
+
+/* ------------------------------------------------------------------- I/O --
+ * GeoTIFF and CSV I/O of the command line (the reference uses GDAL through
+ * GeoTIFFReader / GeoTIFFWriter, GeoTIFF.cpp:20-577, and ReadSamplePoints /
+ * WriteSamplePoints, tiled:1623-1711, 1857-1870).  Errors go to
+ * mkde_last_error(). */
+
+/* Dimensions, geotransform (GDAL order), nodata and block size of band 1
+ * (GeoTIFFReader::GetDimensions / GetGeoTransform / GetNoDataValue /
+ * GetBlockXYSize).  Any output pointer may be NULL. */
+int mkde_tiff_info(const char *path, int64_t *ncols, int64_t *nrows, double *geotransform, double *nodata,
+                   int *has_nodata, int *block_x, int *block_y);
+/* A window of band 1 converted to float (GetRasterBand_NextTile). */
+int mkde_tiff_read(const char *path, int64_t xoff, int64_t yoff, int64_t ncols, int64_t nrows, float *out);
+
+/* Float32 BigTIFF writer, LZW compressed, with georeferencing, nodata and
+ * band statistics (GeoTIFFWriter): open, put windows (WriteGeoTIFF_NextTile),
+ * close.  georef_from (may be NULL) names a GeoTIFF whose GeoKey tags
+ * (projection) are copied. */
+typedef struct mkde_raster_writer mkde_raster_writer;
+int mkde_writer_open(const char *path, int64_t ncols, int64_t nrows, const double *geotransform, float nodata,
+                     const char *georef_from, mkde_raster_writer **out);
+int mkde_writer_put(mkde_raster_writer *w, int64_t xoff, int64_t yoff, int64_t ncols, int64_t nrows,
+                    const float *data);
+int mkde_writer_close(mkde_raster_writer *w, int compress);
+
+/* ReadSamplePoints: header line then x,y,[h,]w per line.  Call with x == NULL
+ * to count (*n receives the count), then with buffers of *n entries. */
+int mkde_csv_read_points(const char *path, int with_h, int sample_weights, int64_t *n, float *x, float *y, float *h,
+                         float *w);
+/* WriteSamplePoints(x, y, h, w) ("x, y, h, w" header, %f fields). */
+int mkde_csv_write_points(const char *path, int64_t n, const float *x, const float *y, const float *h,
+                          const float *w);
+
+/* TIFF LZW codec (TIFF 6.0 section 13); returns the output length or -1. */
+int64_t mkde_lzw_encode(const uint8_t *in, int64_t n, uint8_t *out, int64_t cap);
+int64_t mkde_lzw_decode(const uint8_t *in, int64_t n, uint8_t *out, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -126,8 +126,20 @@ SIGNATURES = {
     "mkde_last_kernel_ms": [_CTX, ctypes.c_char_p, _DP],
     "mkde_run_job": [_CTX, ctypes.POINTER(JobOpts), ctypes.POINTER(JobStats)],
     "mkde_density_stats": [_CTX, ctypes.c_int, _I64P],
+    "mkde_device_memory": [_CTX, _I64P, _I64P],
+    "mkde_tiff_info": [ctypes.c_char_p, _I64P, _I64P, _DP, _DP, _IP, _IP, _IP],
+    "mkde_tiff_read": [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _FP],
+    "mkde_writer_open": [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, _DP, ctypes.c_float, ctypes.c_char_p,
+                         ctypes.POINTER(ctypes.c_void_p)],
+    "mkde_writer_put": [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _FP],
+    "mkde_writer_close": [ctypes.c_void_p, ctypes.c_int],
+    "mkde_csv_read_points": [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _I64P, _FP, _FP, _FP, _FP],
+    "mkde_csv_write_points": [ctypes.c_char_p, ctypes.c_int64, _FP, _FP, _FP, _FP],
+    "mkde_lzw_encode": [_U8P, ctypes.c_int64, _U8P, ctypes.c_int64],
+    "mkde_lzw_decode": [_U8P, ctypes.c_int64, _U8P, ctypes.c_int64],
 }
-_RESTYPES = {"mkde_last_error": ctypes.c_char_p, "mkde_version": ctypes.c_char_p}
+_RESTYPES = {"mkde_last_error": ctypes.c_char_p, "mkde_version": ctypes.c_char_p, "mkde_lzw_encode": ctypes.c_int64,
+             "mkde_lzw_decode": ctypes.c_int64}
 
 _lib = None
 
@@ -355,10 +367,92 @@ class Context:
                 "skipped_blocks", "unused"]
         return dict(zip(keys, out.tolist()))
 
+    def device_memory(self):
+        fr, to = ctypes.c_int64(), ctypes.c_int64()
+        _check(_lib.mkde_device_memory(self._h, ctypes.byref(fr), ctypes.byref(to)))
+        return fr.value, to.value
+
     def last_kernel_ms(self, which="density"):
         v = ctypes.c_double()
         _check(_lib.mkde_last_kernel_ms(self._h, which.encode(), ctypes.byref(v)))
         return v.value
+
+
+# ------------------------------------------------------------------ I/O --
+def _b(path):
+    return os.fsencode(path)
+
+
+def tiff_info(path):
+    """(ncols, nrows, geotransform, nodata or None, (block_x, block_y))."""
+    load_library()
+    nc, nr = ctypes.c_int64(), ctypes.c_int64()
+    gt = np.zeros(6, np.float64)
+    nd = ctypes.c_double()
+    has = ctypes.c_int()
+    bx, by = ctypes.c_int(), ctypes.c_int()
+    _check(_lib.mkde_tiff_info(_b(path), ctypes.byref(nc), ctypes.byref(nr), _p(gt, _DP), ctypes.byref(nd),
+                               ctypes.byref(has), ctypes.byref(bx), ctypes.byref(by)))
+    return nc.value, nr.value, gt, (nd.value if has.value else None), (bx.value, by.value)
+
+
+def tiff_read(path, window=None):
+    """Band 1 as float32 (optionally a window (xoff, yoff, ncols, nrows))."""
+    nc, nr, _, _, _ = tiff_info(path)
+    xo, yo, w, h = window if window else (0, 0, nc, nr)
+    out = np.empty(w * h, np.float32)
+    _check(_lib.mkde_tiff_read(_b(path), xo, yo, w, h, _p(out, _FP)))
+    return out.reshape(h, w)
+
+
+def tiff_write(path, data, geotransform, nodata, georef_from=None, compress=True):
+    load_library()
+    data = _f32(data)
+    nr, nc = data.shape
+    gt = np.ascontiguousarray(geotransform, np.float64)
+    wr = ctypes.c_void_p()
+    _check(_lib.mkde_writer_open(_b(path), nc, nr, _p(gt, _DP), float(nodata),
+                                 _b(georef_from) if georef_from else None, ctypes.byref(wr)))
+    _check(_lib.mkde_writer_put(wr, 0, 0, nc, nr, _p(data, _FP)))
+    _check(_lib.mkde_writer_close(wr, 1 if compress else 0))
+
+
+def csv_read_points(path, with_h=False, sample_weights=False):
+    """ReadSamplePoints: returns (x, y, h, w) float32 arrays."""
+    load_library()
+    n = ctypes.c_int64()
+    _check(_lib.mkde_csv_read_points(_b(path), int(with_h), int(sample_weights), ctypes.byref(n), None, None, None,
+                                     None))
+    x, y, h, w = (np.empty(n.value, np.float32) for _ in range(4))
+    _check(_lib.mkde_csv_read_points(_b(path), int(with_h), int(sample_weights), ctypes.byref(n), _p(x, _FP),
+                                     _p(y, _FP), _p(h, _FP), _p(w, _FP)))
+    return x, y, h, w
+
+
+def csv_write_points(path, x, y, h, w):
+    load_library()
+    x, y, h, w = _f32(x), _f32(y), _f32(h), _f32(w)
+    _check(_lib.mkde_csv_write_points(_b(path), len(x), _p(x, _FP), _p(y, _FP), _p(h, _FP), _p(w, _FP)))
+
+
+def lzw_encode(data):
+    load_library()
+    a = np.frombuffer(bytes(data), np.uint8).copy()
+    out = np.empty(2 * len(a) + 64, np.uint8)
+    k = _lib.mkde_lzw_encode(_p(a, _U8P), len(a), _p(out, _U8P), len(out))
+    if k < 0:
+        raise MkdeError(6, "LZW output buffer too small")
+    return out[:k].tobytes()
+
+
+def lzw_decode(data, size):
+    load_library()
+    a = np.frombuffer(bytes(data), np.uint8).copy()
+    out = np.empty(size, np.uint8)
+    k = _lib.mkde_lzw_decode(_p(a, _U8P), len(a), _p(out, _U8P), size)
+    if k < 0:
+        raise MkdeError(5, "corrupt LZW data")
+    return out[:k].tobytes()
 
 
 def declared_symbols(header=HEADER_PATH):
@@ -366,7 +460,7 @@ def declared_symbols(header=HEADER_PATH):
     import re
 
     txt = open(header).read()
-    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int)\s*(mkde_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int|int64_t)\s*(mkde_\w+)\s*\(", txt, re.M)))
 
 
 def version():

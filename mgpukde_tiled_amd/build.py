@@ -49,7 +49,7 @@ def build(verbose=True):
         _run([HIPCC] + COMMON + HOST_FLAGS + ["-x", "c++", "-c", path, "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC] + DEVICE_FLAGS + ["-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC] + DEVICE_FLAGS + ["-shared", "-fPIC", "-o", tmp] + objs + ["-lz"])
     os.replace(tmp, LIB)
     cli_src = os.path.join(CSRC, "mkde_cli.cpp")
     if os.path.exists(cli_src):

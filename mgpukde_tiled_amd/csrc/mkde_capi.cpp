@@ -181,6 +181,13 @@ struct mkde_ctx {
 
 extern "C" const char *mkde_last_error(void) { return g_err.c_str(); }
 
+// error channel shared with the I/O translation unit
+extern "C" __attribute__((visibility("hidden"))) int mkde_internal_fail(int code, const char *msg)
+{
+    g_err = msg ? msg : "";
+    return code;
+}
+
 extern "C" const char *mkde_version(void) { return "mkde 0.1 (gfx950)"; }
 
 static int set_device(mkde_ctx *c)
@@ -1062,6 +1069,17 @@ extern "C" int mkde_count_density_pairs(mkde_ctx *c, const mkde_grid *tile, cons
     unsigned long long t = 0;
     CHK(copy_d2h(c, &t, c->ucount.p, sizeof t));
     *n_pairs = (int64_t)t;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_device_memory(mkde_ctx *c, int64_t *free_bytes, int64_t *total_bytes)
+{
+    if (!c) return fail(MKDE_E_ARG, "null context");
+    CHK(set_device(c));
+    size_t fr = 0, to = 0;
+    HIPCHK(hipMemGetInfo(&fr, &to));
+    if (free_bytes) *free_bytes = (int64_t)fr;
+    if (total_bytes) *total_bytes = (int64_t)to;
     return MKDE_OK;
 }
 
