@@ -112,6 +112,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--stats", action="store_true", help="report density-kernel classification counters")
+    ap.add_argument("--workload", default="adaptive", choices=["adaptive", "search", "tiles"],
+                    help="adaptive: the benchmark line; search / tiles: measurements of the other workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -120,6 +122,10 @@ def main():
 
     import mgpukde_tiled_amd as M
 
+    if args.workload == "search":
+        return bench_search(args)
+    if args.workload == "tiles":
+        return bench_tiles(args)
     log("workload: generating inputs")
     x, y, w, mask, side = workload(args.scale)
     n = len(x)
@@ -231,6 +237,116 @@ def main():
         out["cpu_baseline"] = None
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def bench_search(args):
+    """Batched bandwidth search: 200,000 points (50% uniform, 50% around 64
+    Gaussian centres with sigma 2% of the extent) on an all-valid 4096^2 mask;
+    LOO log-likelihood of 64 log-spaced bandwidths 0.1x-10x the rule of thumb,
+    edge correction recomputed per candidate."""
+    import cases
+    import mgpukde_tiled_amd as M
+
+    side = int(round(4096 * args.scale))
+    n = int(round(200000 * args.scale * args.scale))
+    x, y = cases.mixture_points(n, float(side), n_centres=64, seed=20260003)
+    mask = cases.all_valid(side, side)
+    ctx = M.Context(0)
+    ctx.points_upload(x, y, None)
+    ctx.mask_upload(M.grid(side, side, 0.0, 0.0, 1.0), mask)
+    h0 = ctx.BandWidth_rule_of_thumb()
+    hc = (np.float32(h0) * (10.0 ** (-1.0 + 2.0 * np.arange(64) / 63.0))).astype(np.float32)
+    for _ in range(args.warmup):
+        ctx.LogLikelihood(hc, edge_correction=True)
+    t = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        L = ctx.LogLikelihood(hc, edge_correction=True)
+        t.append(time.perf_counter() - t0)
+    k = int(np.argmax(L))
+    s = np.sort(L)
+    print(json.dumps({"workload": "search: %d mixture points, %d^2 all-valid mask, 64 candidates" % (n, side),
+                      "ms_per_search": 1e3 * float(np.median(t)), "h_rule_of_thumb": h0,
+                      "argmax": k, "h_selected": float(hc[k]), "top2_margin": float(s[-1] - s[-2]),
+                      "logL": [float(v) for v in L]}), flush=True)
+    ctx.close()
+
+
+def bench_tiles(args):
+    """Precision stress: 10M points with log-normal cluster intensities over
+    1024 centres, a 32768^2 GeoTIFF-style raster at cellsize 1 processed as
+    4096^2 tiles (each tile with its own origin) in one pass, fixed h = 16,
+    EC on.  Reports evaluations/s, HBM GB/s of the raster write, and the
+    max / mean relative error against the FP64 oracle on one 128^2 window in
+    each of several seeded tiles."""
+    import cases
+    import mgpukde_tiled_amd as M
+
+    side = int(round(32768 * args.scale))
+    tile = max(64, int(round(4096 * args.scale)))
+    n = int(round(10_000_000 * args.scale * args.scale))
+    h = 16.0
+    x, y = cases.lognormal_clusters(n, float(side), 1024, 64.0, seed=20260004)
+    w = np.ones(n, np.float32)
+    gt = [0.0, 1.0, 0.0, float(side), 0.0, -1.0]
+    ctx = M.Context(0)
+    ctx.points_upload(x, y, None)
+    ctx.set_bandwidth(h)
+    log("tiles: %d points, %d^2 raster, %d^2 tiles" % (n, side, tile))
+    ctx.mask_upload(M.grid(side, side, 0.0, 0.0, 1.0), cases.all_valid(side, side, 1.0))
+    t0 = time.perf_counter()
+    ew, e_ec = ctx.EdgeCorrectionWeightsExact()
+    ms_ec = 1e3 * (time.perf_counter() - t0)
+    for _ in range(args.warmup + 1):
+        ctx.KernelDensityEstimation_raster(side, side, gt, -9999.0, tile, tile, None, keep_on_device=True)
+    t, ker = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        ctx.KernelDensityEstimation_raster(side, side, gt, -9999.0, tile, tile, None, keep_on_device=True)
+        t.append(time.perf_counter() - t0)
+        ker.append(ctx.last_kernel_ms("density"))
+    den = ctx.tile_download()
+    log("tiles: density %.2f ms (kernel %.2f ms)" % (1e3 * float(np.median(t)), float(np.median(ker))))
+    # exact work: pairs counted tile by tile in each tile's own geometry
+    e_den = 0
+    ntile = (side + tile - 1) // tile
+    for ty in range(ntile):
+        for tx in range(ntile):
+            tc, tr = min(tile, side - tx * tile), min(tile, side - ty * tile)
+            g = M.grid(tc, tr, np.float32(tx * tile), np.float32(side - ty * tile - tr), 1.0)
+            e_den += ctx.count_density_pairs(g, None)
+    kms = float(np.median(ker))
+    res = {"workload": "tiles", "points": n, "raster": side, "tile": tile, "h": h,
+           "ms_per_raster": 1e3 * float(np.median(t)), "kernel_ms": kms, "ms_ec": ms_ec,
+           "E_density": e_den, "E_ec": e_ec, "evals_per_s": e_den / (kms * 1e-3),
+           "raster_write_GBps": 4.0 * side * side / (kms * 1e-3) / 1e9}
+    if not args.no_accuracy:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        rs = np.random.default_rng(20260097)
+        hs = np.full(n, h, np.float32)
+        per_tile = []
+        for _ in range(6):
+            tx, ty = int(rs.integers(0, ntile)), int(rs.integers(0, ntile))
+            tc, tr = min(tile, side - tx * tile), min(tile, side - ty * tile)
+            wc, wr = min(128, tc), min(128, tr)
+            c0 = tx * tile + int(rs.integers(0, tc - wc + 1))
+            r0 = ty * tile + int(rs.integers(0, tr - wr + 1))
+            win = O.grid(wc, wr, float(c0), float(side - r0 - wr), 1.0)
+            sel = O.tile_point_filter(x, y, hs, win)
+            _, o64, _ = O.density_tile(x[sel], y[sel], w[sel], ew[sel], hs[sel], win, None, want64=True)
+            gwin = den[r0:r0 + wr, c0:c0 + wc].astype(np.float64)
+            v = o64 > 0
+            if np.any(v):
+                e = np.abs(gwin[v] - o64[v]) / o64[v]
+                per_tile.append({"tile": [tx, ty], "max_rel_err": float(e.max()), "mean_rel_err": float(e.mean()),
+                                 "cells": int(v.sum())})
+            assert np.all(gwin[~v] == 0.0)
+        res["accuracy"] = per_tile
+        res["max_rel_err"] = max([p["max_rel_err"] for p in per_tile] + [0.0])
+    print(json.dumps(res), flush=True)
+    ctx.close()
 
 
 def ctx_range_total(ctx):

@@ -178,6 +178,24 @@ int mkde_tile_upload(mkde_ctx *ctx, const mkde_grid *tile, const float *mask);
 int mkde_density_tile_resident(mkde_ctx *ctx);
 int mkde_tile_download(mkde_ctx *ctx, float *out);
 
+/* Every tile of a GeoTIFF-style raster in one pass (replaces the tile loop of
+ * main(), tiled:1235-1309 with GeoTIFFReader::GetRasterBand_NextTile,
+ * GeoTIFF.cpp:150-197, and AsciiRasterFromGeoTIFFTile, tiled:2193-2262).
+ * The raster has ncols x nrows cells and geotransform gt[6] (GDAL order);
+ * tiles are tile_cols x tile_rows cells from the top-left, edge tiles
+ * smaller.  Each tile's cells use that tile's origin exactly as the
+ * reference builds it: xll = (float)(gt[0] + gt[1]*xoff), yll =
+ * (float)((gt[3] + gt[5]*yoff) + rows*gt[5]), cellsize = (float)gt[1], so
+ * the result equals mkde_density_tile on each tile.  mask (host, nrows*ncols
+ * floats, row-major) may be NULL (all cells valid); nodata cells keep the mask
+ * value.  mkde_raster_upload leaves the raster as the context's current
+ * tile (for mkde_density_tile_resident / mkde_tile_download / mkde_run_job);
+ * mkde_density_raster uploads, computes and, if out != NULL, copies out. */
+int mkde_raster_upload(mkde_ctx *ctx, int64_t ncols, int64_t nrows, const double *gt, float nodata,
+                       int64_t tile_cols, int64_t tile_rows, const float *mask);
+int mkde_density_raster(mkde_ctx *ctx, int64_t ncols, int64_t nrows, const double *gt, float nodata,
+                        int64_t tile_cols, int64_t tile_rows, const float *mask, float *out);
+
 /* Diagnostic counters of the density kernel (no reference counterpart).
  * enable != 0 zeroes and enables them; counters (8 entries, may be NULL)
  * receives: [0] candidate points classified, [1] FULL and [2] PARTIAL

@@ -123,6 +123,10 @@ SIGNATURES = {
     "mkde_tile_upload": [_CTX, _GP, _FP],
     "mkde_density_tile_resident": [_CTX],
     "mkde_tile_download": [_CTX, _FP],
+    "mkde_raster_upload": [_CTX, ctypes.c_int64, ctypes.c_int64, _DP, ctypes.c_float, ctypes.c_int64, ctypes.c_int64,
+                           _FP],
+    "mkde_density_raster": [_CTX, ctypes.c_int64, ctypes.c_int64, _DP, ctypes.c_float, ctypes.c_int64,
+                            ctypes.c_int64, _FP, _FP],
     "mkde_last_kernel_ms": [_CTX, ctypes.c_char_p, _DP],
     "mkde_run_job": [_CTX, ctypes.POINTER(JobOpts), ctypes.POINTER(JobStats)],
     "mkde_density_stats": [_CTX, ctypes.c_int, _I64P],
@@ -350,6 +354,30 @@ class Context:
         out = np.empty(t.ncols * t.nrows, np.float32)
         _check(_lib.mkde_tile_download(self._h, _p(out, _FP)))
         return out.reshape(t.nrows, t.ncols)
+
+    def raster_upload(self, ncols, nrows, gt, nodata=-9999.0, tile_cols=4096, tile_rows=4096, mask=None):
+        """Make a GeoTIFF-style raster of tiles the current tile (each tile
+        with the origin AsciiRasterFromGeoTIFFTile gives it)."""
+        m = None if mask is None else _f32(mask).reshape(-1)
+        gtv = np.ascontiguousarray(gt, np.float64)
+        assert gtv.size == 6
+        self._tile = grid(int(ncols), int(nrows), 0.0, 0.0, 1.0, nodata)
+        _check(_lib.mkde_raster_upload(self._h, int(ncols), int(nrows), _p(gtv, _DP), float(nodata), int(tile_cols),
+                                       int(tile_rows), _p(m, _FP)))
+
+    def KernelDensityEstimation_raster(self, ncols, nrows, gt, nodata=-9999.0, tile_cols=4096, tile_rows=4096,
+                                       mask=None, keep_on_device=False):
+        """Density of every tile of a raster in one pass (the reference's
+        GeoTIFF tile loop); returns the nrows x ncols raster, or None with
+        keep_on_device (then tile_download() copies it out)."""
+        m = None if mask is None else _f32(mask).reshape(-1)
+        gtv = np.ascontiguousarray(gt, np.float64)
+        assert gtv.size == 6
+        self._tile = grid(int(ncols), int(nrows), 0.0, 0.0, 1.0, nodata)
+        out = None if keep_on_device else np.empty(int(ncols) * int(nrows), np.float32)
+        _check(_lib.mkde_density_raster(self._h, int(ncols), int(nrows), _p(gtv, _DP), float(nodata),
+                                        int(tile_cols), int(tile_rows), _p(m, _FP), _p(out, _FP)))
+        return None if out is None else out.reshape(int(nrows), int(ncols))
 
     def run_job(self, hoption=2, edge_correction=True, optimise=True, alpha=-0.5, recompute_boundary=True):
         """One KDE job on the resident points / mask / tile (the GPU block of

@@ -26,9 +26,11 @@ namespace mkde_dev {
 hipError_t prep_points(hipStream_t, int64_t, const int *, const float *, const float *, const float *, const float *,
                        const float *, float, PtRec *);
 hipError_t bin_stats(hipStream_t, int, const int *, const PtRec *, BinInfo *, double *);
+int64_t raster_blocks(const GridDesc &, const Tiling &);
 hipError_t density_tile(hipStream_t, const GridDesc &, const float *, float *, const PointIndex &,
-                        unsigned long long *, const int *);
-hipError_t block_order(hipStream_t, const GridDesc &, const float *, const PointIndex &, unsigned *, unsigned *, int *);
+                        unsigned long long *, const int *, const Tiling &);
+hipError_t block_order(hipStream_t, const GridDesc &, const float *, const PointIndex &, unsigned *, unsigned *, int *,
+                       const Tiling &);
 hipError_t point_density(hipStream_t, const PointIndex &, int, float *, float);
 hipError_t loo_multi(hipStream_t, const PointIndex &, const float *, const double *, const double *, int, float *);
 hipError_t range_count(hipStream_t, const PointIndex &, int64_t *);
@@ -160,6 +162,7 @@ struct mkde_ctx {
     bool tile_set = false;
     bool tile_has_mask = false;
     GridDesc tg{};
+    Tiling tl{};   // tl.tw == 0: tg is one tile; else tg is a raster of tiles
     DevBuf<float> tmask, tout;
 
     // scratch
@@ -977,6 +980,7 @@ extern "C" int mkde_tile_upload(mkde_ctx *c, const mkde_grid *tile, const float 
     CHK(set_device(c));
     const int64_t nc = tile->ncols * tile->nrows;
     c->tg = to_desc(tile);
+    c->tl = Tiling{};
     CHK(c->tout.ensure(nc));
     c->tile_has_mask = mask != nullptr;
     if (mask) {
@@ -984,6 +988,27 @@ extern "C" int mkde_tile_upload(mkde_ctx *c, const mkde_grid *tile, const float 
         CHK(copy_h2d(c, c->tmask.p, mask, nc * sizeof(float)));
     }
     c->tile_set = true;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_raster_upload(mkde_ctx *c, int64_t ncols, int64_t nrows, const double *gt, float nodata,
+                                  int64_t tile_cols, int64_t tile_rows, const float *mask)
+{
+    if (!c) return fail(MKDE_E_ARG, "null context");
+    if (!gt) return fail(MKDE_E_ARG, "geotransform is NULL");
+    if (tile_cols <= 0 || tile_rows <= 0) return fail(MKDE_E_ARG, "tile size must be positive");
+    for (int k = 0; k < 6; ++k)
+        if (!std::isfinite(gt[k])) return fail(MKDE_E_ARG, "geotransform[%d] is not finite", k);
+    // tiles never exceed the raster; blocks per tile and in total must fit an int
+    if (tile_cols > ncols) tile_cols = ncols;
+    if (tile_rows > nrows) tile_rows = nrows;
+    mkde_grid rg{ncols, nrows, (float)gt[0], (float)(gt[3] + (double)nrows * gt[5]), (float)gt[1], nodata};
+    CHK(check_grid(&rg));
+    const int64_t bpx = (tile_cols + 63) / 64, bpy = (tile_rows + 63) / 64;
+    const int64_t ntx = (ncols + tile_cols - 1) / tile_cols, nty = (nrows + tile_rows - 1) / tile_rows;
+    if (bpx * bpy * ntx * nty > INT32_MAX) return fail(MKDE_E_RANGE, "raster has more than 2^31-1 blocks of 64x64");
+    CHK(mkde_tile_upload(c, &rg, mask));
+    c->tl = Tiling{tile_cols, tile_rows, (int)ntx, (int)bpx, (int)(bpx * bpy), gt[0], gt[1], gt[3], gt[5]};
     return MKDE_OK;
 }
 
@@ -997,16 +1022,16 @@ extern "C" int mkde_density_tile_resident(mkde_ctx *c)
     const float *tm = c->tile_has_mask ? c->tmask.p : nullptr;
     const int *order = nullptr;
     if (c->dt_order_on) {
-        const int64_t nblk = ((c->tg.ncols + 63) / 64) * ((c->tg.nrows + 63) / 64);
+        const int64_t nblk = mkde_dev::raster_blocks(c->tg, c->tl);
         CHK(c->blk_cost.ensure(nblk));
         CHK(c->blk_hist.ensure(32));
         CHK(c->blk_order.ensure(nblk));
-        HIPCHK(mkde_dev::block_order(c->stream, c->tg, tm, I, c->blk_cost.p, c->blk_hist.p, c->blk_order.p));
+        HIPCHK(mkde_dev::block_order(c->stream, c->tg, tm, I, c->blk_cost.p, c->blk_hist.p, c->blk_order.p, c->tl));
         order = c->blk_order.p;
     }
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     HIPCHK(mkde_dev::density_tile(c->stream, c->tg, tm, c->tout.p, I, c->dt_stats_on ? c->dt_stats.p : nullptr,
-                                  order));
+                                  order, c->tl));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     return MKDE_OK;
 }
@@ -1025,6 +1050,17 @@ extern "C" int mkde_tile_download(mkde_ctx *c, float *out)
 extern "C" int mkde_density_tile(mkde_ctx *c, const mkde_grid *tile, const float *mask, float *out)
 {
     CHK(mkde_tile_upload(c, tile, mask));
+    CHK(mkde_density_tile_resident(c));
+    if (out) return mkde_tile_download(c, out);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipEventElapsedTime(&c->last_density_ms, c->ev0, c->ev1));
+    return MKDE_OK;
+}
+
+extern "C" int mkde_density_raster(mkde_ctx *c, int64_t ncols, int64_t nrows, const double *gt, float nodata,
+                                   int64_t tile_cols, int64_t tile_rows, const float *mask, float *out)
+{
+    CHK(mkde_raster_upload(c, ncols, nrows, gt, nodata, tile_cols, tile_rows, mask));
     CHK(mkde_density_tile_resident(c));
     if (out) return mkde_tile_download(c, out);
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -50,6 +50,17 @@ struct PointIndex {
     int64_t n;
 };
 
+// A raster processed as tiles of tw x th cells, each tile with its own origin
+// from the raster geotransform (GDAL order gt0, gt1, gt3, gt5).  Blocks of
+// 64 x 64 cells are numbered tile-major: bpx blocks per tile row, bpt per
+// tile, ntx tiles per raster row (edge tiles leave some blocks empty).
+// tw == 0 means one tile described by its GridDesc.
+struct Tiling {
+    int64_t tw, th;
+    int ntx, bpx, bpt;
+    double gt0, gt1, gt3, gt5;
+};
+
 // Per-row runs of valid cells of a mask (CSR): runs of row r are
 // run_se[run_ptr[r] .. run_ptr[r+1]), each (start, end) inclusive, ascending.
 struct MaskRuns {

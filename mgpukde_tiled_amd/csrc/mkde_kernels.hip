@@ -255,15 +255,61 @@ __global__ void k_bin_stats(int nbins, const int *bin_start, const PtRec *pts, B
 // [6] blocks skipped (no valid cell)
 #define DT_NSTATS 8
 
+// Raster of one or more tiles.  Single tile: g holds the tile geometry.
+// Tiled (tl.tw > 0): g.ncols / g.nrows are the raster size and each tile takes
+// its origin from the raster geotransform as AsciiRasterFromGeoTIFFTile does
+// for a GeoTIFF tile (tiled:2218-2220, GeoTIFF.cpp:190-195):
+//   xll = fl32(gt0 + gt1*xoff), yll = fl32((gt3 + gt5*yoff) + rows*gt5), cs = fl32(gt1).
 struct TileArgs {
     GridDesc g;
-    const float *mask;   // tile values (nullable: all valid)
+    const float *mask;   // raster values (nullable: all valid)
     float *out;
     PointIndex idx;
     unsigned long long *stats;   // nullable
     const int *order;            // nullable: block visiting order (heaviest first)
-    int nbx;                     // blocks per row of the tile
+    int nbx;                     // 64-cell blocks per raster row
+    Tiling tl;
 };
+
+// Geometry of one 64x64 block: global cell range [c0,c1]x[r0,r1], and the
+// origin of the tile it belongs to (block-local cell index = global - off).
+struct BlockGeom {
+    int64_t c0, r0, c1, r1, xoff, yoff;
+    double xll, yll, cs, nrm05;
+};
+
+__device__ __forceinline__ BlockGeom block_geom(const GridDesc &g, const Tiling &tl, int nbx, int blk)
+{
+    BlockGeom b;
+    if (tl.tw > 0) {
+        const int t = blk / tl.bpt, lb = blk - t * tl.bpt;
+        b.xoff = (int64_t)(t % tl.ntx) * tl.tw;
+        b.yoff = (int64_t)(t / tl.ntx) * tl.th;
+        const int64_t tcols = (tl.tw < g.ncols - b.xoff) ? tl.tw : g.ncols - b.xoff;
+        const int64_t trows = (tl.th < g.nrows - b.yoff) ? tl.th : g.nrows - b.yoff;
+        b.c0 = b.xoff + (int64_t)(lb % tl.bpx) * 64;
+        b.r0 = b.yoff + (int64_t)(lb / tl.bpx) * 64;
+        b.c1 = (b.c0 + 63 < b.xoff + tcols - 1) ? b.c0 + 63 : b.xoff + tcols - 1;
+        b.r1 = (b.r0 + 63 < b.yoff + trows - 1) ? b.r0 + 63 : b.yoff + trows - 1;
+        const double g0 = tl.gt0 + tl.gt1 * (double)b.xoff;
+        const double g3 = tl.gt3 + tl.gt5 * (double)b.yoff;
+        b.xll = (double)(float)g0;
+        b.yll = (double)(float)(g3 + (double)trows * tl.gt5);
+        b.cs = (double)(float)tl.gt1;
+        b.nrm05 = (double)trows - 0.5;
+    } else {
+        b.c0 = (int64_t)(blk % nbx) * 64;
+        b.r0 = (int64_t)(blk / nbx) * 64;
+        b.c1 = (b.c0 + 63 < g.ncols - 1) ? b.c0 + 63 : g.ncols - 1;
+        b.r1 = (b.r0 + 63 < g.nrows - 1) ? b.r0 + 63 : g.nrows - 1;
+        b.xoff = b.yoff = 0;
+        b.xll = g.xll;
+        b.yll = g.yll;
+        b.cs = g.cs;
+        b.nrm05 = (double)g.nrows - 0.5;
+    }
+    return b;
+}
 
 template <bool STATS>
 __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
@@ -284,11 +330,12 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const GridDesc g = A.g;
-    const double xll = g.xll, yll = g.yll, cs = g.cs, nrm05 = (double)g.nrows - 0.5;
     const int blk = A.order ? A.order[blockIdx.x] : (int)blockIdx.x;
-    const int64_t c0 = (int64_t)(blk % A.nbx) * 64, r0 = (int64_t)(blk / A.nbx) * 64;
-    const int64_t c1 = (c0 + 63 < g.ncols - 1) ? c0 + 63 : g.ncols - 1;
-    const int64_t r1 = (r0 + 63 < g.nrows - 1) ? r0 + 63 : g.nrows - 1;
+    const BlockGeom B = block_geom(g, A.tl, A.nbx, blk);
+    const double xll = B.xll, yll = B.yll, cs = B.cs, nrm05 = B.nrm05;
+    const int64_t c0 = B.c0, r0 = B.r0, c1 = B.c1, r1 = B.r1;
+    // tile-local indices for cell centres
+    const int64_t lc0 = c0 - B.xoff, lr0 = r0 - B.yoff, lc1 = c1 - B.xoff, lr1 = r1 - B.yoff;
     const int64_t col = c0 + lane;
 
     // validity of this thread's cells; blocks without a valid cell are skipped
@@ -317,9 +364,9 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
     }
 
     // Block extent in cell-centre coordinates (exact N2 floats).
-    const float bx0 = cell_x(xll, cs, c0), bx1 = cell_x(xll, cs, c1);
-    const float by1 = cell_y(yll, cs, nrm05, r0), by0 = cell_y(yll, cs, nrm05, r1);
-    const float cx = cell_x(xll, cs, col);
+    const float bx0 = cell_x(xll, cs, lc0), bx1 = cell_x(xll, cs, lc1);
+    const float by1 = cell_y(yll, cs, nrm05, lr0), by0 = cell_y(yll, cs, nrm05, lr1);
+    const float cx = cell_x(xll, cs, lc0 + lane);
 
     float acc[16], sum[16], comp[16];
 #pragma unroll
@@ -360,10 +407,10 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
                 const float fx = far1(p.x, bx0, bx1), nx = gap1(p.x, bx0, bx1);
 #pragma unroll
                 for (int w = 0; w < 4; ++w) {
-                    const int64_t wr0 = r0 + 16 * w;
-                    const int64_t wr1 = (wr0 + 15 < r1) ? wr0 + 15 : r1;
+                    const int64_t wr0 = lr0 + 16 * w;
+                    const int64_t wr1 = (wr0 + 15 < lr1) ? wr0 + 15 : lr1;
                     int cw = 0;
-                    if (wr0 <= r1) {
+                    if (wr0 <= lr1) {
                         const float wy1 = cell_y(yll, cs, nrm05, wr0), wy0 = cell_y(yll, cs, nrm05, wr1);
                         if (sq2(nx, gap1(p.y, wy0, wy1)) < p.thr) cw = sq2(fx, far1(p.y, wy0, wy1)) < p.thr ? 1 : 2;
                     }
@@ -387,11 +434,11 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const int row = rg * 8 + k + u;
-                        const float cy = cell_y(yll, cs, nrm05, r0 + row);
+                        const float cy = cell_y(yll, cs, nrm05, lr0 + row);
                         const double dy2 = sqd(cy - py);
                         d2v[u] = dy2;
                         (&e4.x)[u] = fast_exp2((float)(dy2 * a)) * coef;
-                        const float cxx = cell_x(xll, cs, c0 + row);   // column index c0 + (same 0..63 slot)
+                        const float cxx = cell_x(xll, cs, lc0 + row);   // column lc0 + (same 0..63 slot)
                         (&x4.x)[u] = fast_exp2((float)(sqd(cxx - px) * a));
                     }
                     *(float4 *)&s_ey[p * DT_EYP + rg * 8 + k] = e4;
@@ -562,28 +609,24 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
 // blocks without a valid cell), then a heaviest-first visiting order by
 // log2 buckets, so the long blocks start early and the launch drains evenly.
 __global__ __launch_bounds__(256) void k_block_cost(GridDesc g, const float *mask, PointIndex I, int nbx,
-                                                    unsigned *cost)
+                                                    Tiling tl, unsigned *cost)
 {
     __shared__ unsigned s_sum[4];
-    __shared__ int s_any;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int blk = blockIdx.x;
-    const double xll = g.xll, yll = g.yll, cs = g.cs, nrm05 = (double)g.nrows - 0.5;
-    const int64_t c0 = (int64_t)(blk % nbx) * 64, r0 = (int64_t)(blk / nbx) * 64;
-    const int64_t c1 = (c0 + 63 < g.ncols - 1) ? c0 + 63 : g.ncols - 1;
-    const int64_t r1 = (r0 + 63 < g.nrows - 1) ? r0 + 63 : g.nrows - 1;
+    const BlockGeom B = block_geom(g, tl, nbx, blk);
     bool v = false;
     if (mask) {
         for (int k = tid; k < 64 * 64; k += 256) {
-            const int64_t r = r0 + k / 64, c = c0 + k % 64;
-            if (r <= r1 && c <= c1 && mask[r * g.ncols + c] != g.nodata) v = true;
+            const int64_t r = B.r0 + k / 64, c = B.c0 + k % 64;
+            if (r <= B.r1 && c <= B.c1 && mask[r * g.ncols + c] != g.nodata) v = true;
         }
     } else {
-        v = true;
+        v = B.c0 <= B.c1 && B.r0 <= B.r1;   // edge tiles leave some blocks empty
     }
     const int any = __syncthreads_or(v);
-    const float bx0 = cell_x(xll, cs, c0), bx1 = cell_x(xll, cs, c1);
-    const float by1 = cell_y(yll, cs, nrm05, r0), by0 = cell_y(yll, cs, nrm05, r1);
+    const float bx0 = cell_x(B.xll, B.cs, B.c0 - B.xoff), bx1 = cell_x(B.xll, B.cs, B.c1 - B.xoff);
+    const float by1 = cell_y(B.yll, B.cs, B.nrm05, B.r0 - B.yoff), by0 = cell_y(B.yll, B.cs, B.nrm05, B.r1 - B.yoff);
     unsigned acc = 0;
     if (any) {
         int bxa, bxb, bya, byb;
@@ -600,7 +643,6 @@ __global__ __launch_bounds__(256) void k_block_cost(GridDesc g, const float *mas
     if (lane == 0) s_sum[wave] = acc;
     __syncthreads();
     if (tid == 0) cost[blk] = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
-    (void)s_any;
 }
 
 __device__ __forceinline__ int cost_bucket(unsigned c)
@@ -859,116 +901,136 @@ struct LooMultiArgs {
 __global__ __launch_bounds__(256) void k_loo_multi(LooMultiArgs A)
 {
     __shared__ int s_bins[256];
-    __shared__ int s_wcnt[4];
-    __shared__ float s_sx[256];
-    __shared__ float s_sy[256];
+    __shared__ int s_pref[257];
+    __shared__ int s_w[4];
+    __shared__ float2 s_xy[256];
+    __shared__ int s_idx[256];
     __shared__ float s_coef[LM_K][256];
-    __shared__ double s_ak[LM_K];
     __shared__ double s_tk[LM_K];
+    __shared__ float s_lo[LM_K], s_hi[LM_K], s_af[LM_K];
     __shared__ float s_red[4][4];
 
     const PointIndex &I = A.idx;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     const int K = A.K;
     const int64_t n = I.n;
     const int64_t q = (int64_t)blockIdx.x * 256 + tid;
     const bool qv = q < n;
     float qx = 0.0f, qy = 0.0f;
     if (qv) {
-        PtRec p = I.pts[q];
+        const PtRec p = I.pts[q];
         qx = p.x;
         qy = p.y;
     }
     if (tid < LM_K) {
-        s_ak[tid] = tid < K ? A.a_k[tid] : 0.0;
-        s_tk[tid] = tid < K ? A.thr_k[tid] : -1.0;
+        const double t = tid < K ? A.thr_k[tid] : -1.0;
+        s_tk[tid] = t;
+        s_lo[tid] = (float)(t * (1.0 - PD_BAND));
+        s_hi[tid] = tid < K ? fmaxf((float)(t * (1.0 + PD_BAND)), FLT_MIN) : -1.0f;
+        s_af[tid] = tid < K ? (float)A.a_k[tid] : 0.0f;
     }
-    float mnx = qv ? qx : FLT_MAX, mxx = qv ? qx : -FLT_MAX, mny = qv ? qy : FLT_MAX, mxy = qv ? qy : -FLT_MAX;
-    for (int d = 32; d >= 1; d >>= 1) {
-        mnx = fminf(mnx, __shfl_xor(mnx, d, WAVE));
-        mxx = fmaxf(mxx, __shfl_xor(mxx, d, WAVE));
-        mny = fminf(mny, __shfl_xor(mny, d, WAVE));
-        mxy = fmaxf(mxy, __shfl_xor(mxy, d, WAVE));
-    }
-    if (lane == 0) {
-        s_red[wave][0] = mnx;
-        s_red[wave][1] = mxx;
-        s_red[wave][2] = mny;
-        s_red[wave][3] = mxy;
-    }
-    __syncthreads();
-    mnx = fminf(fminf(s_red[0][0], s_red[1][0]), fminf(s_red[2][0], s_red[3][0]));
-    mxx = fmaxf(fmaxf(s_red[0][1], s_red[1][1]), fmaxf(s_red[2][1], s_red[3][1]));
-    mny = fminf(fminf(s_red[0][2], s_red[1][2]), fminf(s_red[2][2], s_red[3][2]));
-    mxy = fmaxf(fmaxf(s_red[0][3], s_red[1][3]), fmaxf(s_red[2][3], s_red[3][3]));
+    float mnx, mxx, mny, mxy;
+    block_bbox(qv, qx, qy, s_red, mnx, mxx, mny, mxy);
     const double tmax = s_tk[K - 1];
 
     int bxa, bxb, bya, byb;
-    bin_range(I.bg, mnx, mxx, mny, mxy, sqrt(tmax) * 1.0000001 + 1e-30, bxa, bxb, bya, byb);
+    bin_range(I.bg, mnx, mxx, mny, mxy, sqrt(tmax) * 1.000001 + 1e-6, bxa, bxb, bya, byb);
     const int nbx_r = bxb - bxa + 1;
     const int nb_total = nbx_r * (byb - bya + 1);
 
     float acc[LM_K], sum[LM_K], comp[LM_K];
 #pragma unroll
     for (int k = 0; k < LM_K; ++k) { acc[k] = 0.0f; sum[k] = 0.0f; comp[k] = 0.0f; }
-    int nacc = 0;
 
     for (int base = 0; base < nb_total; base += 256) {
         const int kk = base + tid;
-        bool cand = false;
+        int cntb = 0;
         int b = 0;
         if (kk < nb_total) {
             b = (bya + kk / nbx_r) * I.bg.nbx + (bxa + kk % nbx_r);
-            if (I.bin_start[b + 1] > I.bin_start[b]) cand = box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < tmax;
+            const int c = I.bin_start[b + 1] - I.bin_start[b];
+            if (c > 0 && box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < tmax) cntb = c;
         }
         int nb;
-        int slot = block_compact256(cand, s_wcnt, &nb);
-        if (cand) s_bins[slot] = b;
+        const int slot = block_compact256(cntb > 0, s_w, &nb);
+        if (cntb > 0) s_bins[slot] = b;
         __syncthreads();
-        for (int kb = 0; kb < nb; ++kb) {
-            const int bb = s_bins[kb];
-            const int st = I.bin_start[bb], en = I.bin_start[bb + 1];
-            for (int off = st; off < en; off += 256) {
-                const int cnt = (en - off) < 256 ? (en - off) : 256;
-                if (tid < cnt) {
-                    PtRec p = I.pts[off + tid];
-                    s_sx[tid] = p.x;
-                    s_sy[tid] = p.y;
-#pragma unroll
-                    for (int k = 0; k < LM_K; ++k) s_coef[k][tid] = k < K ? A.coefk[(int64_t)k * n + off + tid] : 0.0f;
+        int T;
+        const int mycnt = tid < nb ? I.bin_start[s_bins[tid] + 1] - I.bin_start[s_bins[tid]] : 0;
+        const int pre = block_exscan256(mycnt, s_w, &T);
+        if (tid < nb) s_pref[tid] = pre;
+        if (tid == 0) s_pref[nb] = T;
+        __syncthreads();
+        for (int g0 = 0; g0 < T; g0 += 256) {
+            const int gidx = g0 + tid;
+            const int cnt = (T - g0) < 256 ? (T - g0) : 256;
+            if (gidx < T) {
+                int lo = 0, hi = nb - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pref[mid] <= gidx) lo = mid;
+                    else hi = mid - 1;
                 }
-                __syncthreads();
-                for (int s = 0; s < cnt; ++s) {
-                    const double d2 = sq2(s_sx[s] - qx, s_sy[s] - qy);
-                    const bool self = (int64_t)(off + s) == q;
+                const int si = I.bin_start[s_bins[lo]] + (gidx - s_pref[lo]);
+                const PtRec p = I.pts[si];
+                s_xy[tid] = make_float2(p.x, p.y);
+                s_idx[tid] = si;
+#pragma unroll
+                for (int k = 0; k < LM_K; ++k) s_coef[k][tid] = k < K ? A.coefk[(int64_t)k * n + si] : 0.0f;
+            }
+            __syncthreads();
+            for (int s0 = 0; s0 < cnt; s0 += 32) {
+                const int s1 = (s0 + 32 < cnt) ? s0 + 32 : cnt;
+                bool band = false;
+                for (int s = s0; s < s1; ++s) {
+                    const float2 sp = s_xy[s];
+                    const float dx = sp.x - qx, dy = sp.y - qy;
+                    const float d2f = __builtin_fmaf(dy, dy, dx * dx);
+                    const bool self = (int64_t)s_idx[s] == q;
 #pragma unroll
                     for (int k = LM_K - 1; k >= 0; --k) {
                         if (k < K) {
-                            const bool in = (d2 < s_tk[k]) && !self;
-                            if (!__any(in)) break;
-                            const float e = fast_exp2((float)(d2 * s_ak[k]));
-                            acc[k] = __builtin_fmaf(in ? e : 0.0f, s_coef[k][s], acc[k]);
+                            const bool maybe = d2f < s_hi[k];
+                            if (!__any(maybe)) break;
+                            const bool in = d2f < s_lo[k];
+                            band |= maybe != in;
+                            const float e = fast_exp2(d2f * s_af[k]);
+                            acc[k] = __builtin_fmaf((in && !self) ? e : 0.0f, s_coef[k][s], acc[k]);
                         }
-                    }
-                    if (++nacc == 32) {
-#pragma unroll
-                        for (int k = 0; k < LM_K; ++k) {
-                            kahan_fold(acc[k], sum[k], comp[k]);
-                            acc[k] = 0.0f;
-                        }
-                        nacc = 0;
                     }
                 }
-                __syncthreads();
+                if (__any(band)) {
+                    // exact FP64 cut-off for this chunk (rare)
+#pragma unroll
+                    for (int k = 0; k < LM_K; ++k) acc[k] = 0.0f;
+                    for (int s = s0; s < s1; ++s) {
+                        const float2 sp = s_xy[s];
+                        const float dx = sp.x - qx, dy = sp.y - qy;
+                        const float d2f = __builtin_fmaf(dy, dy, dx * dx);
+                        const double d2 = sq2(dx, dy);
+                        const bool self = (int64_t)s_idx[s] == q;
+#pragma unroll
+                        for (int k = 0; k < LM_K; ++k) {
+                            if (k < K) {
+                                const bool in = d2 < s_tk[k] && !self;
+                                const float e = fast_exp2(d2f * s_af[k]);
+                                acc[k] = __builtin_fmaf(in ? e : 0.0f, s_coef[k][s], acc[k]);
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < LM_K; ++k) {
+                    kahan_fold(acc[k], sum[k], comp[k]);
+                    acc[k] = 0.0f;
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
 #pragma unroll
-    for (int k = 0; k < LM_K; ++k) {
-        kahan_fold(acc[k], sum[k], comp[k]);
+    for (int k = 0; k < LM_K; ++k)
         if (qv && k < K) A.den1[(int64_t)k * n + q] = sum[k];
-    }
 }
 
 // ==================================================== range candidates ====
@@ -1501,23 +1563,33 @@ hipError_t bin_stats(hipStream_t s, int nbins, const int *bin_start, const PtRec
     return hipGetLastError();
 }
 
-hipError_t density_tile(hipStream_t s, const GridDesc &g, const float *mask, float *out, const PointIndex &idx,
-                        unsigned long long *stats, const int *order)
+// 64 x 64 blocks of a single tile or of a tiled raster (tile-major numbering)
+int64_t raster_blocks(const GridDesc &g, const Tiling &tl)
 {
-    const int nbx = (int)((g.ncols + 63) / 64), nby = (int)((g.nrows + 63) / 64);
-    TileArgs A{g, mask, out, idx, stats, order, nbx};
-    dim3 grid((unsigned)(nbx * nby));
+    if (tl.tw > 0) {
+        const int64_t nty = (g.nrows + tl.th - 1) / tl.th;
+        return (int64_t)tl.ntx * nty * tl.bpt;
+    }
+    return ((g.ncols + 63) / 64) * ((g.nrows + 63) / 64);
+}
+
+hipError_t density_tile(hipStream_t s, const GridDesc &g, const float *mask, float *out, const PointIndex &idx,
+                        unsigned long long *stats, const int *order, const Tiling &tl)
+{
+    const int nbx = (int)((g.ncols + 63) / 64);
+    TileArgs A{g, mask, out, idx, stats, order, nbx, tl};
+    dim3 grid((unsigned)raster_blocks(g, tl));
     if (stats) hipLaunchKernelGGL(k_density_tile<true>, grid, dim3(256), 0, s, A);
     else hipLaunchKernelGGL(k_density_tile<false>, grid, dim3(256), 0, s, A);
     return hipGetLastError();
 }
 
 hipError_t block_order(hipStream_t s, const GridDesc &g, const float *mask, const PointIndex &idx, unsigned *cost,
-                       unsigned *hist, int *order)
+                       unsigned *hist, int *order, const Tiling &tl)
 {
-    const int nbx = (int)((g.ncols + 63) / 64), nby = (int)((g.nrows + 63) / 64);
-    const int nblk = nbx * nby;
-    hipLaunchKernelGGL(k_block_cost, dim3((unsigned)nblk), dim3(256), 0, s, g, mask, idx, nbx, cost);
+    const int nbx = (int)((g.ncols + 63) / 64);
+    const int nblk = (int)raster_blocks(g, tl);
+    hipLaunchKernelGGL(k_block_cost, dim3((unsigned)nblk), dim3(256), 0, s, g, mask, idx, nbx, tl, cost);
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, (int64_t)32, 0.0f, (float *)hist);
     hipLaunchKernelGGL(k_bucket_hist, dim3(nblk_(nblk)), dim3(256), 0, s, nblk, cost, hist);
     hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(64), 0, s, hist);

@@ -101,3 +101,16 @@ def plain_small_bandwidth(x, y):
     sx2 = np.var(x.astype(np.float64), ddof=1)
     sy2 = np.var(y.astype(np.float64), ddof=1)
     return float(np.float32(np.sqrt((sx2 + sy2) / 2.0) * n ** (-1.0 / 6.0)))
+
+
+def lognormal_clusters(n, extent, n_centres=1024, sigma=64.0, seed=20260004):
+    """Cluster centres uniform in [0,E)^2 with log-normal (sigma 1.5)
+    intensities normalised to multinomial counts; N(0, sigma^2) offsets."""
+    r = rng(seed)
+    cx = r.uniform(0.0, extent, n_centres)
+    cy = r.uniform(0.0, extent, n_centres)
+    lam = r.lognormal(0.0, 1.5, n_centres)
+    counts = r.multinomial(n, lam / lam.sum())
+    x = np.repeat(cx, counts) + r.normal(0.0, sigma, n)
+    y = np.repeat(cy, counts) + r.normal(0.0, sigma, n)
+    return clip_domain(x.astype(np.float32), extent), clip_domain(y.astype(np.float32), extent)
