@@ -112,7 +112,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--stats", action="store_true", help="report density-kernel classification counters")
-    ap.add_argument("--workload", default="adaptive", choices=["adaptive", "search", "tiles"],
+    ap.add_argument("--jobs", type=int, default=120, help="serve: number of jobs")
+    ap.add_argument("--slots", type=int, default=4, help="serve: worker slots")
+    ap.add_argument("--qps", type=float, default=0.0, help="serve: arrival rate (0: 4x the sequential rate)")
+    ap.add_argument("--workload", default="adaptive", choices=["adaptive", "search", "tiles", "serve"],
                     help="adaptive: the benchmark line; search / tiles: measurements of the other workloads")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -126,6 +129,8 @@ def main():
         return bench_search(args)
     if args.workload == "tiles":
         return bench_tiles(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     log("workload: generating inputs")
     x, y, w, mask, side = workload(args.scale)
     n = len(x)
@@ -265,9 +270,17 @@ def bench_search(args):
         t.append(time.perf_counter() - t0)
     k = int(np.argmax(L))
     s = np.sort(L)
+    # work (untimed): leave-one-out pairs and EC pairs of every candidate
+    e_loo = e_ec = 0
+    for hk in hc:
+        ctx.set_bandwidth(float(hk))
+        e_loo += ctx_range_total(ctx)[0] - n
+        e_ec += ctx.EdgeCorrectionWeightsExact()[1]
     print(json.dumps({"workload": "search: %d mixture points, %d^2 all-valid mask, 64 candidates" % (n, side),
                       "ms_per_search": 1e3 * float(np.median(t)), "h_rule_of_thumb": h0,
                       "argmax": k, "h_selected": float(hc[k]), "top2_margin": float(s[-1] - s[-2]),
+                      "E_loo": e_loo, "E_ec": e_ec,
+                      "evals_per_s": (e_loo + e_ec) / float(np.median(t)),
                       "logL": [float(v) for v in L]}), flush=True)
     ctx.close()
 
@@ -307,6 +320,12 @@ def bench_tiles(args):
         ker.append(ctx.last_kernel_ms("density"))
     den = ctx.tile_download()
     log("tiles: density %.2f ms (kernel %.2f ms)" % (1e3 * float(np.median(t)), float(np.median(ker))))
+    dstats = None
+    if args.stats:
+        ctx.density_stats(True)
+        ctx.KernelDensityEstimation_raster(side, side, gt, -9999.0, tile, tile, None, keep_on_device=True)
+        dstats = ctx.density_stats(False)
+        log("density stats: %s" % dstats)
     # exact work: pairs counted tile by tile in each tile's own geometry
     e_den = 0
     ntile = (side + tile - 1) // tile
@@ -319,7 +338,7 @@ def bench_tiles(args):
     res = {"workload": "tiles", "points": n, "raster": side, "tile": tile, "h": h,
            "ms_per_raster": 1e3 * float(np.median(t)), "kernel_ms": kms, "ms_ec": ms_ec,
            "E_density": e_den, "E_ec": e_ec, "evals_per_s": e_den / (kms * 1e-3),
-           "raster_write_GBps": 4.0 * side * side / (kms * 1e-3) / 1e9}
+           "raster_write_GBps": 4.0 * side * side / (kms * 1e-3) / 1e9, "density_stats": dstats}
     if not args.no_accuracy:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -328,11 +347,14 @@ def bench_tiles(args):
         hs = np.full(n, h, np.float32)
         per_tile = []
         for _ in range(6):
-            tx, ty = int(rs.integers(0, ntile)), int(rs.integers(0, ntile))
-            tc, tr = min(tile, side - tx * tile), min(tile, side - ty * tile)
-            wc, wr = min(128, tc), min(128, tr)
-            c0 = tx * tile + int(rs.integers(0, tc - wc + 1))
-            r0 = ty * tile + int(rs.integers(0, tr - wr + 1))
+            wc, wr = min(128, tile, side), min(128, tile, side)
+            # centre the window on a random point, so it lands where the density is
+            k = int(rs.integers(0, n))
+            c0 = int(min(max(int(x[k]) - wc // 2, 0), side - wc))
+            r0 = int(min(max(side - 1 - int(y[k]) - wr // 2, 0), side - wr))
+            tx, ty = c0 // tile, r0 // tile
+            c0 = min(c0, tx * tile + min(tile, side - tx * tile) - wc)
+            r0 = min(r0, ty * tile + min(tile, side - ty * tile) - wr)
             win = O.grid(wc, wr, float(c0), float(side - r0 - wr), 1.0)
             sel = O.tile_point_filter(x, y, hs, win)
             _, o64, _ = O.density_tile(x[sel], y[sel], w[sel], ew[sel], hs[sel], win, None, want64=True)
@@ -347,6 +369,120 @@ def bench_tiles(args):
         res["max_rel_err"] = max([p["max_rel_err"] for p in per_tile] + [0.0])
     print(json.dumps(res), flush=True)
     ctx.close()
+
+
+def serve_jobs(count, seed=20260005, max_points=5_000_000, max_side=8192):
+    """The serving stream's jobs: points log-uniform 1e3 - 5e6 (uniform or
+    Thomas-clustered, 50/50), square rasters with side log-uniform 256 - 8192
+    at cellsize 1, random disk or rectangle masks with holes, 50% fixed
+    rule-of-thumb, 30% adaptive (Abramson, alpha -0.5), 20% bandwidth search
+    over 16 candidates; edge correction on."""
+    import cases
+    import mgpukde_tiled_amd as M
+
+    r = np.random.default_rng(seed)
+    jobs = []
+    for i in range(count):
+        n = int(round(np.exp(r.uniform(np.log(1e3), np.log(max_points)))))
+        side = int(round(np.exp(r.uniform(np.log(256), np.log(max_side)))))
+        ext = float(side)
+        if r.random() < 0.5:
+            x, y = cases.uniform_points(n, ext, seed=seed + 7 * i)
+        else:
+            mu = 500
+            x, y, _ = cases.thomas_points(max(1, n // mu), mu, ext / 200.0, ext, seed=seed + 7 * i, weights=False)
+        u = r.random()
+        hopt = 0 if u < 0.5 else (2 if u < 0.8 else M.MKDE_H_SEARCH)
+        holes = int(r.integers(0, 8))
+        if r.random() < 0.5:
+            mask = cases.disk_mask(side, side, side * r.uniform(0.3, 0.5), n_holes=holes,
+                                   hole_min=max(2, side // 40), hole_max=max(3, side // 10), seed=seed + 7 * i)
+        else:
+            mask = cases.rect_mask(side, side, n_holes=holes, hole_min=max(2, side // 40),
+                                   hole_max=max(3, side // 10), seed=seed + 7 * i)
+        jobs.append(dict(x=x, y=y, g=M.grid(side, side, 0.0, 0.0, 1.0), mask=mask, hoption=hopt, n=len(x),
+                         side=side))
+    return jobs
+
+
+def bench_serve(args):
+    """Serving stream: the same job list run one job at a time (one slot,
+    back to back) and served (several slots, Poisson arrivals at a fixed QPS);
+    jobs/s, p50/p99 latency and bit-equality of every result."""
+    import hashlib
+    import threading
+
+    import mgpukde_tiled_amd as M
+
+    log("serve: generating %d jobs" % args.jobs)
+    jobs = serve_jobs(args.jobs)
+    log("serve: %d points, %.3g cells in total" % (sum(j["n"] for j in jobs), sum(j["side"] ** 2 for j in jobs)))
+
+    def submit(srv, j):
+        return srv.submit(j["x"], j["y"], j["g"], j["mask"], hoption=j["hoption"], edge_correction=True,
+                          optimise=False, alpha=-0.5, n_candidates=16, cand_lo=0.1, cand_hi=10.0)
+
+    def digest(a):
+        return hashlib.blake2b(a.tobytes(), digest_size=16).hexdigest()
+
+    # warm the slots' buffers and code paths on a few jobs (untimed)
+    with M.Server(slots=1) as srv:
+        for j in jobs[:2]:
+            srv.wait(submit(srv, j))
+    # one job at a time
+    seq = []
+    with M.Server(slots=1) as srv:
+        t0 = time.perf_counter()
+        for j in jobs:
+            res, out = srv.wait(submit(srv, j))
+            assert res["status"] == 0, res["message"]
+            seq.append((res, digest(out)))
+        t_seq = time.perf_counter() - t0
+    seq_rate = len(jobs) / t_seq
+    log("serve: sequential %.2f jobs/s (%.1f s)" % (seq_rate, t_seq))
+    qps = args.qps if args.qps > 0 else 4.0 * seq_rate
+    arr = np.cumsum(np.random.default_rng(20260006).exponential(1.0 / qps, len(jobs)))
+    results = [None] * len(jobs)
+    with M.Server(slots=args.slots) as srv:
+        tickets = [None] * len(jobs)
+        submitted = threading.Event()
+
+        def collect():
+            for i in range(len(jobs)):
+                while tickets[i] is None:
+                    submitted.wait(0.001)
+                res, out = srv.wait(tickets[i])
+                results[i] = (res, digest(out))
+
+        th = threading.Thread(target=collect)
+        th.start()
+        t0 = time.perf_counter()
+        sub_t = []
+        for i, j in enumerate(jobs):
+            dt = t0 + arr[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            sub_t.append(time.perf_counter() - t0)
+            tickets[i] = submit(srv, j)
+        th.join()
+        info = srv.info()
+    ends = [sub_t[i] + results[i][0]["latency_ms"] * 1e-3 for i in range(len(jobs))]
+    t_served = max(ends) - sub_t[0]
+    served_rate = len(jobs) / t_served
+    lat = np.array([r[0]["latency_ms"] for r in results])
+    run = np.array([r[0]["run_ms"] for r in results])
+    seq_run = np.array([r[0]["run_ms"] for r in seq])
+    same = sum(1 for a, b in zip(seq, results) if a[1] == b[1] and a[0]["h_sel"] == b[0]["h_sel"])
+    out = {"workload": "serve", "jobs": len(jobs), "slots": args.slots, "qps": qps,
+           "sequential_jobs_per_s": seq_rate, "served_jobs_per_s": served_rate, "speedup": served_rate / seq_rate,
+           "served_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+           "served_run_ms": {"p50": float(np.percentile(run, 50)), "p99": float(np.percentile(run, 99))},
+           "sequential_run_ms": {"p50": float(np.percentile(seq_run, 50)), "p99": float(np.percentile(seq_run, 99))},
+           "identical_results": same, "peak_committed_bytes": info["peak_committed"], "budget_bytes": info["budget"],
+           "mix": {"fixed": sum(j["hoption"] == 0 for j in jobs), "adaptive": sum(j["hoption"] == 2 for j in jobs),
+                   "search": sum(j["hoption"] == 3 for j in jobs)}}
+    print(json.dumps(out), flush=True)
+    return out
 
 
 def ctx_range_total(ctx):

@@ -40,7 +40,9 @@ typedef enum {
     MKDE_E_NOMEM = 3,   /* device or host allocation failed */
     MKDE_E_STATE = 4,   /* a required input (points, mask, bandwidths) is not set */
     MKDE_E_IO = 5,      /* file read/write/format error */
-    MKDE_E_RANGE = 6    /* a size exceeds a documented maximum */
+    MKDE_E_RANGE = 6,   /* a size exceeds a documented maximum */
+    MKDE_E_PENDING = 7, /* a served job has not finished yet */
+    MKDE_E_CANCELLED = 8 /* a served job was cancelled before it started */
 } mkde_status;
 
 /* Raster geometry of a mask, density tile or output raster.  Mirrors the
@@ -66,6 +68,13 @@ const char *mkde_version(void);
 int mkde_ctx_create(int device, mkde_ctx **out);
 int mkde_ctx_destroy(mkde_ctx *ctx);
 int mkde_synchronize(mkde_ctx *ctx);
+
+/* Device memory held by a context's buffers (bytes; buffers grow to the
+ * largest job seen and are reused), and a release of all of them (the
+ * context keeps its stream and forgets points, mask, tile and bandwidths).
+ * No reference counterpart: used by the job server's HBM admission. */
+int mkde_ctx_memory(mkde_ctx *ctx, int64_t *bytes);
+int mkde_ctx_release(mkde_ctx *ctx);
 
 /* SamplePoints upload (AllocateDeviceSamplePoints + CopyToDeviceSamplePoints,
  * tiled:1713-1805).  Builds the device-resident cell-binned point index that
@@ -191,6 +200,10 @@ int mkde_tile_download(mkde_ctx *ctx, float *out);
  * value.  mkde_raster_upload leaves the raster as the context's current
  * tile (for mkde_density_tile_resident / mkde_tile_download / mkde_run_job);
  * mkde_density_raster uploads, computes and, if out != NULL, copies out. */
+/* The density tile is the resident edge-correction mask raster (same grid,
+ * same values; no second upload).  A later mkde_mask_upload unsets it. */
+int mkde_tile_from_mask(mkde_ctx *ctx);
+
 int mkde_raster_upload(mkde_ctx *ctx, int64_t ncols, int64_t nrows, const double *gt, float nodata,
                        int64_t tile_cols, int64_t tile_rows, const float *mask);
 int mkde_density_raster(mkde_ctx *ctx, int64_t ncols, int64_t nrows, const double *gt, float nodata,
@@ -285,6 +298,72 @@ int mkde_csv_write_points(const char *path, int64_t n, const float *x, const flo
 /* TIFF LZW codec (TIFF 6.0 section 13); returns the output length or -1. */
 int64_t mkde_lzw_encode(const uint8_t *in, int64_t n, uint8_t *out, int64_t cap);
 int64_t mkde_lzw_decode(const uint8_t *in, int64_t n, uint8_t *out, int64_t cap);
+
+/* --------------------------------------------------------------- serving --
+ * A job server on one device (no reference counterpart: the reference runs
+ * one job per process).  `slots` worker threads each own a context (its own
+ * stream; device buffers grow to the largest job seen and are reused).  Jobs
+ * start in submission order when a slot is free and HBM admission allows it:
+ * the device bytes held by all slots, plus the footprint bound
+ * (mkde_job_footprint) of every running job, plus the new job's bound must
+ * not exceed hbm_budget (<= 0: 80% of the device's free memory at creation).
+ * A job whose bound alone exceeds the budget is refused at submit with
+ * MKDE_E_NOMEM; when the next job does not fit and no job runs, idle slots
+ * release their buffers first.  Jobs share nothing on the device, so a job's
+ * result is the same alone and in any mix.  The caller's input and output
+ * buffers must stay valid until the job has finished (mkde_server_wait). */
+typedef struct mkde_server mkde_server;
+
+#define MKDE_H_SEARCH 3   /* hoption: argmax LOO log-likelihood over candidates */
+
+typedef struct {
+    int64_t n;
+    const float *x, *y;    /* n points */
+    const float *w;        /* sample weights, NULL: all 1 */
+    const float *h;        /* hoption -1: n bandwidths, else NULL */
+    mkde_grid grid;        /* edge-correction mask and density raster geometry */
+    const float *mask;     /* nrows*ncols values (nodata: outside); NULL: all cells
+                              valid, allowed only without edge correction */
+    int hoption;           /* -1, 0, 1, 2 as mkde_run_job, or MKDE_H_SEARCH */
+    int edge_correction;
+    int optimise;          /* hoption 2: Hooke-Jeeves first */
+    float alpha;           /* hoption 2 without optimise */
+    int n_candidates;      /* MKDE_H_SEARCH: K <= 1024 bandwidths */
+    float cand_lo, cand_hi; /* h_k = (float)(h0*lo*(hi/lo)^(k/(K-1))), h0 = rule of thumb */
+    float *out;            /* nrows*ncols density raster (host) */
+} mkde_serve_job;
+
+typedef struct {
+    int status;            /* MKDE_OK, the job's error, or MKDE_E_CANCELLED */
+    char message[256];     /* error text of a failed job */
+    int slot;              /* worker slot that ran it (-1: never started) */
+    double queue_ms, run_ms, latency_ms;   /* submit -> start -> finish */
+    float h0;              /* rule-of-thumb bandwidth */
+    float h_sel, alpha_sel; /* selected bandwidth (MLE / search / Hooke-Jeeves h0) and alpha */
+    int64_t footprint;     /* admission bound used for the job, bytes */
+} mkde_serve_result;
+
+typedef struct {
+    int slots;
+    int64_t budget;        /* bytes */
+    int64_t held;          /* device bytes held by all slots now */
+    int64_t peak_committed; /* max over time of held + bounds of running jobs */
+    int64_t submitted, finished, refused, releases;
+    int64_t queued, running;
+} mkde_server_info;
+
+/* Admission bound of a job's device memory (bytes) on a fresh context. */
+int64_t mkde_job_footprint(int64_t n, int64_t ncols, int64_t nrows, int hoption, int optimise, int n_candidates);
+int mkde_server_create(int device, int slots, int64_t hbm_budget, mkde_server **out);
+int mkde_server_submit(mkde_server *s, const mkde_serve_job *job, int64_t *ticket);
+/* MKDE_OK: the job has finished (res filled, ticket retired); MKDE_E_PENDING:
+ * not within timeout_ms (< 0: wait without limit). */
+int mkde_server_wait(mkde_server *s, int64_t ticket, double timeout_ms, mkde_serve_result *res);
+/* Cancels a job that has not started (MKDE_E_STATE if it has). */
+int mkde_server_cancel(mkde_server *s, int64_t ticket);
+int mkde_server_info_get(mkde_server *s, mkde_server_info *info);
+/* Cancels queued jobs, waits for running ones, frees everything. */
+int mkde_server_destroy(mkde_server *s);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,7 @@ HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "mkde.h")
 
 MKDE_OK = 0
 STATUS_NAMES = {1: "MKDE_E_ARG", 2: "MKDE_E_HIP", 3: "MKDE_E_NOMEM", 4: "MKDE_E_STATE", 5: "MKDE_E_IO",
-                6: "MKDE_E_RANGE"}
+                6: "MKDE_E_RANGE", 7: "MKDE_E_PENDING", 8: "MKDE_E_CANCELLED"}
 
 
 class MkdeError(RuntimeError):
@@ -87,6 +87,66 @@ _I64P = ctypes.POINTER(ctypes.c_int64)
 _I32P = ctypes.POINTER(ctypes.c_int32)
 _U8P = ctypes.POINTER(ctypes.c_uint8)
 _IP = ctypes.POINTER(ctypes.c_int)
+class ServeJob(ctypes.Structure):
+    """mkde_serve_job (see include/mkde.h)."""
+
+    _fields_ = [
+        ("n", ctypes.c_int64),
+        ("x", ctypes.POINTER(ctypes.c_float)),
+        ("y", ctypes.POINTER(ctypes.c_float)),
+        ("w", ctypes.POINTER(ctypes.c_float)),
+        ("h", ctypes.POINTER(ctypes.c_float)),
+        ("grid", Grid),
+        ("mask", ctypes.POINTER(ctypes.c_float)),
+        ("hoption", ctypes.c_int),
+        ("edge_correction", ctypes.c_int),
+        ("optimise", ctypes.c_int),
+        ("alpha", ctypes.c_float),
+        ("n_candidates", ctypes.c_int),
+        ("cand_lo", ctypes.c_float),
+        ("cand_hi", ctypes.c_float),
+        ("out", ctypes.POINTER(ctypes.c_float)),
+    ]
+
+
+class ServeResult(ctypes.Structure):
+    """mkde_serve_result (see include/mkde.h)."""
+
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("message", ctypes.c_char * 256),
+        ("slot", ctypes.c_int),
+        ("queue_ms", ctypes.c_double),
+        ("run_ms", ctypes.c_double),
+        ("latency_ms", ctypes.c_double),
+        ("h0", ctypes.c_float),
+        ("h_sel", ctypes.c_float),
+        ("alpha_sel", ctypes.c_float),
+        ("footprint", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["message"] = self.message.decode(errors="replace")
+        return d
+
+
+class ServerInfo(ctypes.Structure):
+    """mkde_server_info (see include/mkde.h)."""
+
+    _fields_ = [(k, ctypes.c_int if k == "slots" else ctypes.c_int64) for k in
+                ("slots", "budget", "held", "peak_committed", "submitted", "finished", "refused", "releases",
+                 "queued", "running")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+MKDE_H_SEARCH = 3
+MKDE_E_NOMEM = 3
+MKDE_E_PENDING = 7
+MKDE_E_CANCELLED = 8
+
 _GP = ctypes.POINTER(Grid)
 _CTX = ctypes.c_void_p
 
@@ -141,9 +201,19 @@ SIGNATURES = {
     "mkde_csv_write_points": [ctypes.c_char_p, ctypes.c_int64, _FP, _FP, _FP, _FP],
     "mkde_lzw_encode": [_U8P, ctypes.c_int64, _U8P, ctypes.c_int64],
     "mkde_lzw_decode": [_U8P, ctypes.c_int64, _U8P, ctypes.c_int64],
+    "mkde_ctx_memory": [_CTX, _I64P],
+    "mkde_ctx_release": [_CTX],
+    "mkde_tile_from_mask": [_CTX],
+    "mkde_job_footprint": [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "mkde_server_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)],
+    "mkde_server_submit": [ctypes.c_void_p, ctypes.POINTER(ServeJob), _I64P],
+    "mkde_server_wait": [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ServeResult)],
+    "mkde_server_cancel": [ctypes.c_void_p, ctypes.c_int64],
+    "mkde_server_info_get": [ctypes.c_void_p, ctypes.POINTER(ServerInfo)],
+    "mkde_server_destroy": [ctypes.c_void_p],
 }
 _RESTYPES = {"mkde_last_error": ctypes.c_char_p, "mkde_version": ctypes.c_char_p, "mkde_lzw_encode": ctypes.c_int64,
-             "mkde_lzw_decode": ctypes.c_int64}
+             "mkde_lzw_decode": ctypes.c_int64, "mkde_job_footprint": ctypes.c_int64}
 
 _lib = None
 
@@ -355,6 +425,20 @@ class Context:
         _check(_lib.mkde_tile_download(self._h, _p(out, _FP)))
         return out.reshape(t.nrows, t.ncols)
 
+    def memory(self):
+        """Device bytes held by this context's buffers."""
+        v = ctypes.c_int64()
+        _check(_lib.mkde_ctx_memory(self._h, ctypes.byref(v)))
+        return v.value
+
+    def release(self):
+        _check(_lib.mkde_ctx_release(self._h))
+
+    def tile_from_mask(self):
+        """The density tile is the resident EC mask raster."""
+        self._tile = self.mask_grid
+        _check(_lib.mkde_tile_from_mask(self._h))
+
     def raster_upload(self, ncols, nrows, gt, nodata=-9999.0, tile_cols=4096, tile_rows=4096, mask=None):
         """Make a GeoTIFF-style raster of tiles the current tile (each tile
         with the origin AsciiRasterFromGeoTIFFTile gives it)."""
@@ -404,6 +488,82 @@ class Context:
         v = ctypes.c_double()
         _check(_lib.mkde_last_kernel_ms(self._h, which.encode(), ctypes.byref(v)))
         return v.value
+
+
+# -------------------------------------------------------------- serving --
+def job_footprint(n, ncols, nrows, hoption=0, optimise=False, n_candidates=0):
+    load_library()
+    return int(_lib.mkde_job_footprint(int(n), int(ncols), int(nrows), int(hoption), 1 if optimise else 0,
+                                       int(n_candidates)))
+
+
+class Server:
+    """The job server (mkde_server_*): worker slots with their own contexts
+    and streams, FIFO start with HBM admission.  submit() returns a ticket;
+    the arrays of a job are kept referenced here until wait() returns it."""
+
+    def __init__(self, device=0, slots=4, hbm_budget=0):
+        load_library()
+        h = ctypes.c_void_p()
+        _check(_lib.mkde_server_create(int(device), int(slots), int(hbm_budget), ctypes.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def submit(self, x, y, g, mask=None, out=None, w=None, h=None, hoption=0, edge_correction=True, optimise=False,
+               alpha=-0.5, n_candidates=16, cand_lo=0.1, cand_hi=10.0):
+        """Queue one job; returns its ticket.  The density lands in `out`
+        (allocated here when None) once wait() reports it finished."""
+        x, y = _f32(x), _f32(y)
+        w = None if w is None else _f32(w)
+        h = None if h is None else _f32(h)
+        m = None if mask is None else _f32(mask).reshape(-1)
+        if out is None:
+            out = np.empty((g.nrows, g.ncols), np.float32)
+        assert out.dtype == np.float32 and out.flags.c_contiguous and out.size == g.ncols * g.nrows
+        j = ServeJob(len(x), _p(x, _FP), _p(y, _FP), _p(w, _FP), _p(h, _FP), g, _p(m, _FP), int(hoption),
+                     1 if edge_correction else 0, 1 if optimise else 0, float(alpha), int(n_candidates),
+                     float(cand_lo), float(cand_hi), _p(out, _FP))
+        t = ctypes.c_int64()
+        _check(_lib.mkde_server_submit(self._h, ctypes.byref(j), ctypes.byref(t)))
+        self._keep[t.value] = (x, y, w, h, m, out)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1.0):
+        """(result dict, output raster) once the job has finished; None while
+        it is still pending after timeout_ms."""
+        r = ServeResult()
+        st = _lib.mkde_server_wait(self._h, int(ticket), float(timeout_ms), ctypes.byref(r))
+        if st == MKDE_E_PENDING:
+            return None
+        _check(st)
+        keep = self._keep.pop(ticket)
+        return r.as_dict(), keep[5]
+
+    def cancel(self, ticket):
+        _check(_lib.mkde_server_cancel(self._h, int(ticket)))
+
+    def info(self):
+        i = ServerInfo()
+        _check(_lib.mkde_server_info_get(self._h, ctypes.byref(i)))
+        return i.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.mkde_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------------ I/O --

@@ -26,7 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + 
 HOST_FLAGS = ["-D__HIP_PLATFORM_AMD__=1", "-I/opt/rocm/include"]
 
 HIP_SOURCES = ["mkde_kernels.hip"]
-HOST_SOURCES = ["mkde_capi.cpp", "mkde_tiff.cpp", "mkde_driver.cpp"]
+HOST_SOURCES = ["mkde_capi.cpp", "mkde_tiff.cpp", "mkde_serve.cpp"]
 
 
 def _run(cmd):
@@ -49,7 +49,7 @@ def build(verbose=True):
         _run([HIPCC] + COMMON + HOST_FLAGS + ["-x", "c++", "-c", path, "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC] + DEVICE_FLAGS + ["-shared", "-fPIC", "-o", tmp] + objs + ["-lz"])
+    _run([HIPCC] + DEVICE_FLAGS + ["-shared", "-fPIC", "-o", tmp] + objs + ["-lz", "-pthread"])
     os.replace(tmp, LIB)
     cli_src = os.path.join(CSRC, "mkde_cli.cpp")
     if os.path.exists(cli_src):

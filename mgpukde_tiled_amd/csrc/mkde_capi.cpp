@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
@@ -27,10 +28,14 @@ hipError_t prep_points(hipStream_t, int64_t, const int *, const float *, const f
                        const float *, float, PtRec *);
 hipError_t bin_stats(hipStream_t, int, const int *, const PtRec *, BinInfo *, double *);
 int64_t raster_blocks(const GridDesc &, const Tiling &);
+hipError_t density_plan(hipStream_t, const GridDesc &, const float *, const PointIndex &, const Tiling &,
+                        const DensityWork &);
+hipError_t density_items(hipStream_t, int, const DensityWork &, int, int4 *, int4 *, int *);
 hipError_t density_tile(hipStream_t, const GridDesc &, const float *, float *, const PointIndex &,
-                        unsigned long long *, const int *, const Tiling &);
-hipError_t block_order(hipStream_t, const GridDesc &, const float *, const PointIndex &, unsigned *, unsigned *, int *,
-                       const Tiling &);
+                        unsigned long long *, const int *, const int4 *, const unsigned *, double *, int,
+                        const Tiling &);
+hipError_t density_merge(hipStream_t, const GridDesc &, const Tiling &, const float *, int, const int4 *,
+                         const double *, float *);
 hipError_t point_density(hipStream_t, const PointIndex &, int, float *, float);
 hipError_t loo_multi(hipStream_t, const PointIndex &, const float *, const double *, const double *, int, float *);
 hipError_t range_count(hipStream_t, const PointIndex &, int64_t *);
@@ -79,17 +84,25 @@ static int fail(int code, const char *fmt, ...)
         if (_s != MKDE_OK) return _s; \
     } while (0)
 
+// Device bytes held by the context whose entry point runs on this thread
+// (set by set_device); DevBuf allocations are charged to it.
+static thread_local std::atomic<int64_t> *t_acct = nullptr;
+
 template <class T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    std::atomic<int64_t> *acct = nullptr;   // the owning context's device-byte counter
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            if (acct) *acct -= (int64_t)(n * sizeof(T));
+        }
         p = nullptr;
         n = 0;
     }
@@ -104,6 +117,8 @@ struct DevBuf {
             return fail(MKDE_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", cnt * sizeof(T), hipGetErrorString(e));
         }
         n = cnt;
+        acct = t_acct;
+        if (acct) *acct += (int64_t)(cnt * sizeof(T));
         return MKDE_OK;
     }
 };
@@ -117,6 +132,7 @@ struct RunsHost {
 
 struct mkde_ctx {
     int device = 0;
+    std::atomic<int64_t> dev_bytes{0};   // device memory held by the buffers below
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_density_ms = 0.0f, last_ec_ms = 0.0f, last_points_ms = 0.0f;
@@ -161,6 +177,7 @@ struct mkde_ctx {
     // tile
     bool tile_set = false;
     bool tile_has_mask = false;
+    bool tile_is_ec_mask = false;   // the density tile is the resident EC mask raster
     GridDesc tg{};
     Tiling tl{};   // tl.tw == 0: tg is one tile; else tg is a raster of tiles
     DevBuf<float> tmask, tout;
@@ -173,13 +190,19 @@ struct mkde_ctx {
     DevBuf<double> partial, dsum, gval;
     DevBuf<float> coefk, ewk, den1k;
     DevBuf<double> ak, tk;
+    DevBuf<float> hsave, ewsave;   // bandwidths / EC weights saved across a likelihood scan
     // density kernel counters (mkde_density_stats)
     bool dt_stats_on = false;
     DevBuf<unsigned long long> dt_stats;
-    // heaviest-first block order of the density kernel
+    // work list of the density kernel: heavy blocks split into parts,
+    // items visited heaviest first
     bool dt_order_on = true;
-    DevBuf<unsigned> blk_cost, blk_hist;
-    DevBuf<int> blk_order;
+    DevBuf<unsigned> blk_cost, blk_parts, blk_item0, blk_slot0, blk_sidx, blk_sidx2, blk_chunks, blk_counts,
+        blk_hist;
+    DevBuf<unsigned long long> blk_total;
+    DevBuf<int4> dt_items, dt_splits;
+    DevBuf<int> dt_order;
+    DevBuf<double> dt_partials;
 };
 
 extern "C" const char *mkde_last_error(void) { return g_err.c_str(); }
@@ -195,6 +218,7 @@ extern "C" const char *mkde_version(void) { return "mkde 0.1 (gfx950)"; }
 
 static int set_device(mkde_ctx *c)
 {
+    t_acct = &c->dev_bytes;
     HIPCHK(hipSetDevice(c->device));
     return MKDE_OK;
 }
@@ -234,6 +258,35 @@ extern "C" int mkde_ctx_destroy(mkde_ctx *c)
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    t_acct = nullptr;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_ctx_memory(mkde_ctx *c, int64_t *bytes)
+{
+    if (!c || !bytes) return fail(MKDE_E_ARG, "null argument");
+    *bytes = c->dev_bytes.load();
+    return MKDE_OK;
+}
+
+extern "C" int mkde_ctx_release(mkde_ctx *c)
+{
+    if (!c) return fail(MKDE_E_ARG, "null context");
+    CHK(set_device(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // keep the stream, events and settings; drop every buffer and all state
+    const int dev = c->device;
+    hipStream_t s = c->stream;
+    hipEvent_t e0 = c->ev0, e1 = c->ev1;
+    const bool order_on = c->dt_order_on;
+    c->~mkde_ctx();
+    new (c) mkde_ctx();
+    c->device = dev;
+    c->stream = s;
+    c->ev0 = e0;
+    c->ev1 = e1;
+    c->dt_order_on = order_on;
+    t_acct = &c->dev_bytes;
     return MKDE_OK;
 }
 
@@ -510,6 +563,7 @@ extern "C" int mkde_mask_upload(mkde_ctx *c, const mkde_grid *g, const float *va
     c->mask_set = true;
     c->nboundary = -1;
     c->dist_set = false;
+    if (c->tile_is_ec_mask) c->tile_set = c->tile_is_ec_mask = false;
     HIPCHK(hipStreamSynchronize(c->stream));
     return MKDE_OK;
 }
@@ -588,6 +642,8 @@ extern "C" int mkde_set_bandwidths(mkde_ctx *c, const float *h)
 {
     CHK(need_points(c));
     if (!h) return fail(MKDE_E_ARG, "h is NULL");
+    for (int64_t i = 0; i < c->n; ++i)
+        if (!std::isfinite(h[i])) return fail(MKDE_E_ARG, "bandwidth of point %lld is not finite", (long long)i);
     CHK(copy_h2d(c, c->h.p, h, c->n * sizeof(float)));
     c->h_fixed = 0.0f;
     c->h_set = true;
@@ -760,7 +816,7 @@ extern "C" int mkde_loo_loglik(mkde_ctx *c, int K, const float *h_cand, int ec_o
     float hmax = h_cand[ord[K - 1]];
     const float saved_hf = c->h_fixed;
     const bool saved_hset = c->h_set;
-    DevBuf<float> hsave;
+    DevBuf<float> &hsave = c->hsave;
     CHK(hsave.ensure(n));
     HIPCHK(hipMemcpyAsync(hsave.p, c->h.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     c->h_fixed = hmax;
@@ -856,7 +912,7 @@ extern "C" int mkde_loo_loglik_adaptive(mkde_ctx *c, int K, const float *h, cons
     if (K <= 0 || !h || !alpha || !logL) return fail(MKDE_E_ARG, "mkde_loo_loglik_adaptive: bad arguments");
     if (ec_on) CHK(need_mask(c));
     const int64_t n = c->n;
-    DevBuf<float> hsave, ewsave;
+    DevBuf<float> &hsave = c->hsave, &ewsave = c->ewsave;
     CHK(hsave.ensure(n));
     CHK(ewsave.ensure(n));
     HIPCHK(hipMemcpyAsync(hsave.p, c->h.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
@@ -981,12 +1037,27 @@ extern "C" int mkde_tile_upload(mkde_ctx *c, const mkde_grid *tile, const float 
     const int64_t nc = tile->ncols * tile->nrows;
     c->tg = to_desc(tile);
     c->tl = Tiling{};
+    c->tile_is_ec_mask = false;
     CHK(c->tout.ensure(nc));
     c->tile_has_mask = mask != nullptr;
     if (mask) {
         CHK(c->tmask.ensure(nc));
         CHK(copy_h2d(c, c->tmask.p, mask, nc * sizeof(float)));
     }
+    c->tile_set = true;
+    return MKDE_OK;
+}
+
+extern "C" int mkde_tile_from_mask(mkde_ctx *c)
+{
+    if (!c) return fail(MKDE_E_ARG, "null context");
+    CHK(need_mask(c));
+    CHK(set_device(c));
+    c->tg = c->mg;
+    c->tl = Tiling{};
+    CHK(c->tout.ensure(c->mg.ncols * c->mg.nrows));
+    c->tile_has_mask = true;
+    c->tile_is_ec_mask = true;
     c->tile_set = true;
     return MKDE_OK;
 }
@@ -1019,19 +1090,41 @@ extern "C" int mkde_density_tile_resident(mkde_ctx *c)
     if (!c->tile_set) return fail(MKDE_E_STATE, "no tile uploaded");
     CHK(prep(c));
     PointIndex I = index_of(c);
-    const float *tm = c->tile_has_mask ? c->tmask.p : nullptr;
+    const float *tm = c->tile_is_ec_mask ? c->mask.p : c->tile_has_mask ? c->tmask.p : nullptr;
     const int *order = nullptr;
+    int n_items = 0, n_split = 0;
     if (c->dt_order_on) {
         const int64_t nblk = mkde_dev::raster_blocks(c->tg, c->tl);
         CHK(c->blk_cost.ensure(nblk));
+        CHK(c->blk_parts.ensure(nblk));
+        CHK(c->blk_item0.ensure(nblk));
+        CHK(c->blk_slot0.ensure(nblk));
+        CHK(c->blk_sidx.ensure(nblk));
+        CHK(c->blk_sidx2.ensure(nblk));
+        CHK(c->blk_chunks.ensure((nblk + 1023) / 1024));
+        CHK(c->blk_counts.ensure(3));
         CHK(c->blk_hist.ensure(32));
-        CHK(c->blk_order.ensure(nblk));
-        HIPCHK(mkde_dev::block_order(c->stream, c->tg, tm, I, c->blk_cost.p, c->blk_hist.p, c->blk_order.p, c->tl));
-        order = c->blk_order.p;
+        CHK(c->blk_total.ensure(1));
+        DensityWork W{c->blk_cost.p, c->blk_parts.p,  c->blk_item0.p, c->blk_slot0.p, c->blk_sidx.p,
+                      c->blk_sidx2.p, c->blk_chunks.p, c->blk_total.p, c->blk_counts.p, c->blk_hist.p};
+        HIPCHK(mkde_dev::density_plan(c->stream, c->tg, tm, I, c->tl, W));
+        unsigned cnt[3];
+        CHK(copy_d2h(c, cnt, c->blk_counts.p, sizeof cnt));
+        n_items = (int)cnt[0];
+        n_split = (int)cnt[2];
+        CHK(c->dt_items.ensure(n_items));
+        CHK(c->dt_order.ensure(n_items));
+        CHK(c->dt_splits.ensure(n_split));
+        CHK(c->dt_partials.ensure((size_t)cnt[1] * 4096));
+        HIPCHK(mkde_dev::density_items(c->stream, (int)nblk, W, n_items, c->dt_items.p, c->dt_splits.p,
+                                       c->dt_order.p));
+        order = c->dt_order.p;
     }
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     HIPCHK(mkde_dev::density_tile(c->stream, c->tg, tm, c->tout.p, I, c->dt_stats_on ? c->dt_stats.p : nullptr,
-                                  order, c->tl));
+                                  order, c->dt_items.p, c->blk_cost.p, c->dt_partials.p, n_items, c->tl));
+    HIPCHK(mkde_dev::density_merge(c->stream, c->tg, c->tl, tm, n_split, c->dt_splits.p, c->dt_partials.p,
+                                   c->tout.p));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     return MKDE_OK;
 }

@@ -61,6 +61,18 @@ struct Tiling {
     double gt0, gt1, gt3, gt5;
 };
 
+// Device buffers of the density kernel's work list (per 64x64 block unless
+// noted): candidate counts, split parts, scans of parts / split parts / split
+// flags, scan scratch, the candidate total and the {items, slots, splits}
+// counts read back by the host.
+struct DensityWork {
+    unsigned *cost, *parts, *item0, *slot0, *sidx, *sidx2;
+    unsigned *chunks;              // (nblk + 1023) / 1024
+    unsigned long long *total;     // 1
+    unsigned *counts;              // 3
+    unsigned *hist;                // 32
+};
+
 // Per-row runs of valid cells of a mask (CSR): runs of row r are
 // run_se[run_ptr[r] .. run_ptr[r+1]), each (start, end) inclusive, ascending.
 struct MaskRuns {

@@ -29,6 +29,7 @@
 //                float-float prefix sums); FP64 only for log-likelihood sums.
 #include <hip/hip_runtime.h>
 #include <float.h>
+#include <algorithm>
 #include <stdint.h>
 
 #include "mkde_device.h"
@@ -260,13 +261,23 @@ __global__ void k_bin_stats(int nbins, const int *bin_start, const PtRec *pts, B
 // its origin from the raster geotransform as AsciiRasterFromGeoTIFFTile does
 // for a GeoTIFF tile (tiled:2218-2220, GeoTIFF.cpp:190-195):
 //   xll = fl32(gt0 + gt1*xoff), yll = fl32((gt3 + gt5*yoff) + rows*gt5), cs = fl32(gt1).
+// A work item is one part of a 64x64 block: {block, part j, parts K, partial
+// slot (K > 1) or -1}.  A block whose candidate count exceeds the split
+// threshold is cut into K parts by candidate ordinal (the position of a point
+// in the block's bin-order candidate sequence): part j takes ordinals
+// [C j / K, C (j+1) / K) of the C candidates and writes FP64 partial sums to
+// its slot; k_density_merge adds the K partials in part order.  Items are
+// visited heaviest first (order[]).
 struct TileArgs {
     GridDesc g;
     const float *mask;   // raster values (nullable: all valid)
     float *out;
     PointIndex idx;
     unsigned long long *stats;   // nullable
-    const int *order;            // nullable: block visiting order (heaviest first)
+    const int *order;            // nullable: items in raster order, one per block
+    const int4 *items;           // with order: work items
+    const unsigned *cost;        // with order: candidate count of each block
+    double *partials;            // 4096 per slot
     int nbx;                     // 64-cell blocks per raster row
     Tiling tl;
 };
@@ -327,10 +338,22 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
     __shared__ __attribute__((aligned(16))) float s_ex[DT_SB * 64];
     __shared__ __attribute__((aligned(16))) float s_ey[DT_SB * DT_EYP];
     __shared__ __attribute__((aligned(16))) double s_dy2[DT_SB * DT_DYP];
+    __shared__ float s_ccx[64], s_ccy[64];   // cell-centre x of the block's columns, y of its rows
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const GridDesc g = A.g;
-    const int blk = A.order ? A.order[blockIdx.x] : (int)blockIdx.x;
+    int4 item = make_int4((int)blockIdx.x, 0, 1, -1);
+    if (A.order) item = A.items[A.order[blockIdx.x]];
+    const int blk = item.x;
+    // candidate-ordinal window of this part
+    int64_t win0 = 0, win1 = INT64_MAX;
+    if (item.z > 1) {
+        // the first and last parts are open-ended, so the parts always cover
+        // the whole candidate sequence
+        const int64_t C = A.cost[blk];
+        if (item.y > 0) win0 = C * item.y / item.z;
+        if (item.y < item.z - 1) win1 = C * (item.y + 1) / item.z;
+    }
     const BlockGeom B = block_geom(g, A.tl, A.nbx, blk);
     const double xll = B.xll, yll = B.yll, cs = B.cs, nrm05 = B.nrm05;
     const int64_t c0 = B.c0, r0 = B.r0, c1 = B.c1, r1 = B.r1;
@@ -367,6 +390,8 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
     const float bx0 = cell_x(xll, cs, lc0), bx1 = cell_x(xll, cs, lc1);
     const float by1 = cell_y(yll, cs, nrm05, lr0), by0 = cell_y(yll, cs, nrm05, lr1);
     const float cx = cell_x(xll, cs, lc0 + lane);
+    if (tid < 64) s_ccx[tid] = cx;
+    else if (tid < 128) s_ccy[tid - 64] = cell_y(yll, cs, nrm05, lr0 + (tid - 64));
 
     float acc[16], sum[16], comp[16];
 #pragma unroll
@@ -379,7 +404,8 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
     const int nbx_r = bxb - bxa + 1;
     const int nb_total = nbx_r * (byb - bya + 1);
 
-    int qn = 0;   // queue length (uniform)
+    int qn = 0;       // queue length (uniform)
+    int64_t ord = 0;  // candidate ordinal at the start of the next candidate bin (uniform)
 
     auto process = [&](int cnt) {
         // 1a. point parameters, block-FULL points first (stable partition, so
@@ -421,30 +447,20 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
             }
         }
         __syncthreads();
-        // 1b. tables: thread -> point (tid & 31), rows / columns 8*(tid>>5) .. +7
+        // 1b. tables: lane -> row / column slot, wave -> 8 points; consecutive
+        //     lanes write consecutive LDS words (no bank conflicts)
         {
-            const int p = tid & (DT_SB - 1), rg = tid >> 5;
-            if (p < cnt) {
-                const float py = s_py[p], px = s_px[p], coef = s_coef[p];
-                const double a = s_a[p];
-#pragma unroll
-                for (int k = 0; k < 8; k += 4) {
-                    float4 e4, x4;
-                    double d2v[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int row = rg * 8 + k + u;
-                        const float cy = cell_y(yll, cs, nrm05, lr0 + row);
-                        const double dy2 = sqd(cy - py);
-                        d2v[u] = dy2;
-                        (&e4.x)[u] = fast_exp2((float)(dy2 * a)) * coef;
-                        const float cxx = cell_x(xll, cs, lc0 + row);   // column lc0 + (same 0..63 slot)
-                        (&x4.x)[u] = fast_exp2((float)(sqd(cxx - px) * a));
-                    }
-                    *(float4 *)&s_ey[p * DT_EYP + rg * 8 + k] = e4;
-                    *(float4 *)&s_ex[p * 64 + rg * 8 + k] = x4;
-                    *(double2 *)&s_dy2[p * DT_DYP + rg * 8 + k] = make_double2(d2v[0], d2v[1]);
-                    *(double2 *)&s_dy2[p * DT_DYP + rg * 8 + k + 2] = make_double2(d2v[2], d2v[3]);
+            const float cyr = s_ccy[lane], cxr = s_ccx[lane];
+#pragma unroll 2
+            for (int q = 0; q < DT_SB / 4; ++q) {
+                const int p = wave * (DT_SB / 4) + q;   // wave-uniform
+                if (p < cnt) {
+                    const float py = s_py[p], px = s_px[p], coef = s_coef[p];
+                    const double a = s_a[p];
+                    const double dy2 = sqd(cyr - py);
+                    s_dy2[p * DT_DYP + lane] = dy2;
+                    s_ey[p * DT_EYP + lane] = fast_exp2((float)(dy2 * a)) * coef;
+                    s_ex[p * 64 + lane] = fast_exp2((float)(sqd(cxr - px) * a));
                 }
             }
         }
@@ -528,7 +544,13 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
         __syncthreads();
         for (int kb = 0; kb < nb; ++kb) {
             const int bb = s_bins[kb];
-            const int st = I.bin_start[bb], en = I.bin_start[bb + 1];
+            const int bst = I.bin_start[bb], ben = I.bin_start[bb + 1];
+            // clip the bin's points to this part's ordinal window
+            const int64_t o0 = ord, o1 = ord + (ben - bst);
+            ord = o1;
+            if (o1 <= win0 || o0 >= win1) continue;
+            const int st = bst + (int)(win0 > o0 ? win0 - o0 : 0);
+            const int en = ben - (int)(o1 > win1 ? o1 - win1 : 0);
             for (int off = st; off < en; off += 256) {
                 const int i = off + tid;
                 int cls = 0;   // 0 empty, 1 full, 2 partial
@@ -579,8 +601,13 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
     }
     if (qn > 0) process(qn);
 
-    // write back: nodata cells keep the mask value
-    if (col <= c1) {
+    // write back: nodata cells keep the mask value; parts of a split block
+    // leave FP64 partials (sum - compensation) for k_density_merge
+    if (item.z > 1) {
+        double *pp = A.partials + (int64_t)item.w * 4096;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) pp[(wave * 16 + j) * 64 + lane] = (double)sum[j] - (double)comp[j];
+    } else if (col <= c1) {
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             const int64_t row = r0 + wave * 16 + j;
@@ -651,12 +678,6 @@ __device__ __forceinline__ int cost_bucket(unsigned c)
     return c == 0 ? 31 : (__clz(c) < 31 ? __clz(c) : 30);
 }
 
-__global__ void k_bucket_hist(int nblk, const unsigned *cost, unsigned *hist)
-{
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < nblk) atomicAdd(&hist[cost_bucket(cost[b])], 1u);
-}
-
 __global__ void k_bucket_scan(unsigned *hist)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -669,10 +690,157 @@ __global__ void k_bucket_scan(unsigned *hist)
     }
 }
 
-__global__ void k_bucket_fill(int nblk, const unsigned *cost, unsigned *offs, int *order)
+// Sum of the K partials of each split block in part order (FP64), rounded
+// once to FP32; nodata cells keep the mask value.  splits[k] = {block, slot0, K}.
+__global__ __launch_bounds__(256) void k_density_merge(GridDesc g, Tiling tl, int nbx, const float *mask,
+                                                       const int4 *splits, const double *partials, float *out)
 {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < nblk) order[atomicAdd(&offs[cost_bucket(cost[b])], 1u)] = b;
+    const int4 sp = splits[blockIdx.x];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const BlockGeom B = block_geom(g, tl, nbx, sp.x);
+    const int64_t col = B.c0 + lane;
+    if (col > B.c1) return;
+#pragma unroll 4
+    for (int j = 0; j < 16; ++j) {
+        const int64_t row = B.r0 + wave * 16 + j;
+        if (row > B.r1) break;
+        const int64_t ix = row * g.ncols + col;
+        const double *pp = partials + (int64_t)sp.y * 4096 + (wave * 16 + j) * 64 + lane;
+        double t = 0.0;
+        for (int k = 0; k < sp.z; ++k) t += pp[(int64_t)k * 4096];
+        const bool valid = !mask || mask[ix] != g.nodata;
+        out[ix] = valid ? (float)t : mask[ix];
+    }
+}
+
+// ---- work list of the density kernel
+//
+// cost[b] (candidates of block b) -> split threshold T = max(DT_TMIN,
+// ceil(total / DT_TARGET)), parts[b] = cost > T ? ceil(cost / T) : 1, exclusive
+// scans for item and slot offsets, items[], splits[], and a heaviest-first
+// order of the items by cost / parts.
+#define DT_TMIN 2048u
+#define DT_TARGET 4096ull
+
+__global__ void k_cost_total(int n, const unsigned *cost, unsigned long long *total)
+{
+    __shared__ unsigned long long s_w[4];
+    unsigned long long acc = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += cost[i];
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, WAVE);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(total, s_w[0] + s_w[1] + s_w[2] + s_w[3]);
+}
+
+__device__ __forceinline__ unsigned split_threshold(unsigned long long total)
+{
+    const unsigned long long t = (total + DT_TARGET - 1) / DT_TARGET;
+    return t > DT_TMIN ? (unsigned)(t < 0xffffffffull ? t : 0xffffffffull) : DT_TMIN;
+}
+
+__global__ void k_parts(int n, const unsigned *cost, const unsigned long long *total, unsigned *parts,
+                        unsigned *sparts, unsigned *sflag)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const unsigned T = split_threshold(*total);
+    const unsigned c = cost[b];
+    const unsigned k = c > T ? (unsigned)(((unsigned long long)c + T - 1) / T) : 1u;
+    parts[b] = k;
+    sparts[b] = k > 1 ? k : 0u;
+    sflag[b] = k > 1 ? 1u : 0u;
+}
+
+__global__ void k_items_fill(int n, const unsigned *parts, const unsigned *item0, const unsigned *slot0,
+                             const unsigned *sidx, int4 *items, int4 *splits)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const int k = (int)parts[b];
+    const int i0 = (int)item0[b];
+    for (int j = 0; j < k; ++j) items[i0 + j] = make_int4(b, j, k, k > 1 ? (int)slot0[b] + j : -1);
+    if (k > 1) splits[sidx[b]] = make_int4(b, (int)slot0[b], k, 0);
+}
+
+__global__ void k_item_hist(int n, const int4 *items, const unsigned *cost, unsigned *hist)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int4 it = items[i];
+        atomicAdd(&hist[cost_bucket(cost[it.x] / (unsigned)it.z)], 1u);
+    }
+}
+
+__global__ void k_item_order(int n, const int4 *items, const unsigned *cost, unsigned *offs, int *order)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int4 it = items[i];
+        order[atomicAdd(&offs[cost_bucket(cost[it.x] / (unsigned)it.z)], 1u)] = i;
+    }
+}
+
+// ---- exclusive scan of unsigned values (1024 per workgroup, two levels)
+__global__ __launch_bounds__(256) void k_scan_local(int n, const unsigned *in, unsigned *out, unsigned *chunk_sums)
+{
+    __shared__ unsigned s_w[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t base = (int64_t)blockIdx.x * 1024 + tid * 4;
+    unsigned v[4], t = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        t += v[k];
+    }
+    unsigned x = t;   // inclusive wave scan of thread totals
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned y = __shfl_up(x, d, WAVE);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    unsigned woff = 0;
+    for (int w = 0; w < wave; ++w) woff += s_w[w];
+    unsigned run = woff + x - t;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (tid == 255) chunk_sums[blockIdx.x] = woff + x;
+}
+
+__global__ __launch_bounds__(256) void k_scan_chunks(int nchunks, unsigned *chunk_sums, unsigned *total)
+{
+    __shared__ unsigned s_w[4];
+    __shared__ unsigned s_carry;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nchunks; base += 256) {
+        const unsigned t = base + tid < nchunks ? chunk_sums[base + tid] : 0u;
+        unsigned x = t;
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned y = __shfl_up(x, d, WAVE);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) s_w[wave] = x;
+        __syncthreads();
+        unsigned woff = s_carry;
+        for (int w = 0; w < wave; ++w) woff += s_w[w];
+        if (base + tid < nchunks) chunk_sums[base + tid] = woff + x - t;
+        __syncthreads();
+        if (tid == 255) s_carry = woff + x;
+        __syncthreads();
+    }
+    if (tid == 0) *total = s_carry;
+}
+
+__global__ void k_scan_add(int n, unsigned *out, const unsigned *chunk_sums)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += chunk_sums[i / 1024];
 }
 
 // ==================================================== density at points ====
@@ -1573,27 +1741,66 @@ int64_t raster_blocks(const GridDesc &g, const Tiling &tl)
     return ((g.ncols + 63) / 64) * ((g.nrows + 63) / 64);
 }
 
-hipError_t density_tile(hipStream_t s, const GridDesc &g, const float *mask, float *out, const PointIndex &idx,
-                        unsigned long long *stats, const int *order, const Tiling &tl)
+static void scan_u32(hipStream_t s, int n, const unsigned *in, unsigned *out, unsigned *chunks, unsigned *total)
+{
+    const int nchunks = (n + 1023) / 1024;
+    hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nchunks), dim3(256), 0, s, n, in, out, chunks);
+    hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(256), 0, s, nchunks, chunks, total);
+    hipLaunchKernelGGL(k_scan_add, dim3(nblk_(n)), dim3(256), 0, s, n, out, chunks);
+}
+
+// Costs, split parts and the three scans; W.counts receives {items, slots, splits}.
+hipError_t density_plan(hipStream_t s, const GridDesc &g, const float *mask, const PointIndex &idx, const Tiling &tl,
+                        const DensityWork &W)
 {
     const int nbx = (int)((g.ncols + 63) / 64);
-    TileArgs A{g, mask, out, idx, stats, order, nbx, tl};
-    dim3 grid((unsigned)raster_blocks(g, tl));
+    const int nblk = (int)raster_blocks(g, tl);
+    hipLaunchKernelGGL(k_block_cost, dim3((unsigned)nblk), dim3(256), 0, s, g, mask, idx, nbx, tl, W.cost);
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, (int64_t)2, 0.0f, (float *)W.total);
+    const unsigned nred = (unsigned)std::min<int64_t>(1024, ((int64_t)nblk + 255) / 256);
+    hipLaunchKernelGGL(k_cost_total, dim3(nred), dim3(256), 0, s, nblk, W.cost, W.total);
+    hipLaunchKernelGGL(k_parts, dim3(nblk_(nblk)), dim3(256), 0, s, nblk, W.cost, W.total, W.parts, W.item0,
+                       W.sidx);
+    // in place is not allowed: parts -> item0 scan reads parts; sparts / sflag
+    // were written to item0 / sidx and are scanned into slot0 / sidx2 below
+    scan_u32(s, nblk, W.item0, W.slot0, W.chunks, W.counts + 1);
+    scan_u32(s, nblk, W.sidx, W.sidx2, W.chunks, W.counts + 2);
+    scan_u32(s, nblk, W.parts, W.item0, W.chunks, W.counts + 0);
+    return hipGetLastError();
+}
+
+hipError_t density_items(hipStream_t s, int nblk, const DensityWork &W, int n_items, int4 *items, int4 *splits,
+                         int *order)
+{
+    hipLaunchKernelGGL(k_items_fill, dim3(nblk_(nblk)), dim3(256), 0, s, nblk, W.parts, W.item0, W.slot0, W.sidx2,
+                       items, splits);
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, (int64_t)32, 0.0f, (float *)W.hist);
+    hipLaunchKernelGGL(k_item_hist, dim3(nblk_(n_items)), dim3(256), 0, s, n_items, items, W.cost, W.hist);
+    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(64), 0, s, W.hist);
+    hipLaunchKernelGGL(k_item_order, dim3(nblk_(n_items)), dim3(256), 0, s, n_items, items, W.cost, W.hist, order);
+    return hipGetLastError();
+}
+
+// order == nullptr: one item per block in raster order; else n_items items.
+hipError_t density_tile(hipStream_t s, const GridDesc &g, const float *mask, float *out, const PointIndex &idx,
+                        unsigned long long *stats, const int *order, const int4 *items, const unsigned *cost,
+                        double *partials, int n_items, const Tiling &tl)
+{
+    const int nbx = (int)((g.ncols + 63) / 64);
+    TileArgs A{g, mask, out, idx, stats, order, items, cost, partials, nbx, tl};
+    dim3 grid((unsigned)(order ? n_items : raster_blocks(g, tl)));
     if (stats) hipLaunchKernelGGL(k_density_tile<true>, grid, dim3(256), 0, s, A);
     else hipLaunchKernelGGL(k_density_tile<false>, grid, dim3(256), 0, s, A);
     return hipGetLastError();
 }
 
-hipError_t block_order(hipStream_t s, const GridDesc &g, const float *mask, const PointIndex &idx, unsigned *cost,
-                       unsigned *hist, int *order, const Tiling &tl)
+hipError_t density_merge(hipStream_t s, const GridDesc &g, const Tiling &tl, const float *mask, int n_split,
+                         const int4 *splits, const double *partials, float *out)
 {
+    if (n_split <= 0) return hipSuccess;
     const int nbx = (int)((g.ncols + 63) / 64);
-    const int nblk = (int)raster_blocks(g, tl);
-    hipLaunchKernelGGL(k_block_cost, dim3((unsigned)nblk), dim3(256), 0, s, g, mask, idx, nbx, tl, cost);
-    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, (int64_t)32, 0.0f, (float *)hist);
-    hipLaunchKernelGGL(k_bucket_hist, dim3(nblk_(nblk)), dim3(256), 0, s, nblk, cost, hist);
-    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(64), 0, s, hist);
-    hipLaunchKernelGGL(k_bucket_fill, dim3(nblk_(nblk)), dim3(256), 0, s, nblk, cost, hist, order);
+    hipLaunchKernelGGL(k_density_merge, dim3((unsigned)n_split), dim3(256), 0, s, g, tl, nbx, mask, splits, partials,
+                       out);
     return hipGetLastError();
 }
 

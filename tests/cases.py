@@ -75,6 +75,23 @@ def disk_mask(ncols, nrows, radius_cells, n_holes=0, hole_min=50, hole_max=400, 
     return m
 
 
+def rect_mask(ncols, nrows, n_holes=0, hole_min=50, hole_max=400, seed=20260002, nodata=NODATA):
+    """A centred rectangle of valid cells covering 40-90% of each side, minus holes."""
+    r = rng(seed + 2000)
+    m = np.full((nrows, ncols), nodata, np.float32)
+    w = max(1, int(ncols * r.uniform(0.4, 0.9)))
+    h = max(1, int(nrows * r.uniform(0.4, 0.9)))
+    c0, r0 = (ncols - w) // 2, (nrows - h) // 2
+    m[r0:r0 + h, c0:c0 + w] = 1.0
+    for _ in range(n_holes):
+        hw = int(r.integers(hole_min, hole_max + 1))
+        hh = int(r.integers(hole_min, hole_max + 1))
+        hc = int(r.integers(0, max(1, ncols - hw)))
+        hr = int(r.integers(0, max(1, nrows - hh)))
+        m[hr:hr + hh, hc:hc + hw] = nodata
+    return m
+
+
 def all_valid(ncols, nrows, value=0.0):
     return np.full((nrows, ncols), value, np.float32)
 

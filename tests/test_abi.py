@@ -47,3 +47,24 @@ def test_product_does_not_reference_the_oracle():
 
 def test_version_string():
     assert "gfx950" in M.version()
+
+
+def test_job_footprint_bound_shape():
+    # linear in points and cells; the leave-one-out options add their candidate arrays
+    base = M.job_footprint(0, 0, 0)
+    assert base > 0
+    assert M.job_footprint(1000, 100, 100) < M.job_footprint(2000, 100, 100) < M.job_footprint(2000, 200, 200)
+    for hopt, opt in ((1, False), (M.MKDE_H_SEARCH, False), (2, True)):
+        assert M.job_footprint(10000, 64, 64, hopt, opt, 16) > M.job_footprint(10000, 64, 64, 0)
+    assert M.job_footprint(10000, 64, 64, 2, False) == M.job_footprint(10000, 64, 64, 0)
+    assert M.job_footprint(-1, 1, 1) == -1
+
+
+def test_server_without_a_device_fails_loudly():
+    try:
+        srv = M.Server(slots=2)
+    except M.MkdeError as e:
+        assert e.code == 2 and "device" in str(e)   # MKDE_E_HIP, no silent CPU fallback
+    else:
+        srv.close()
+        pytest.skip("a HIP device is present")

@@ -96,3 +96,54 @@ def test_raster_argument_errors(ctx):
         ctx.KernelDensityEstimation_raster(10, 10, [0, float("nan"), 0, 10, 0, -1], -9999.0, 4, 4, None)
     with pytest.raises(M.MkdeError):
         ctx.KernelDensityEstimation_raster(10, 10, [0, -1.0, 0, 10, 0, -1], -9999.0, 4, 4, None)
+
+
+def test_split_blocks_dense_cluster(orc, ctx):
+    # 24k points inside 40 x 40 cells: every block near them has more
+    # candidates than the split threshold, so its work is cut into parts whose
+    # FP64 partials are merged; the result stays within 1e-5 of the FP64
+    # oracle, is deterministic, and the raster path agrees bit for bit
+    r = np.random.default_rng(5)
+    n = 24000
+    x = (r.normal(60.0, 8.0, n)).astype(np.float32)
+    y = (r.normal(70.0, 8.0, n)).astype(np.float32)
+    w = r.uniform(0.5, 2.0, n).astype(np.float32)
+    hs = r.uniform(2.0, 9.0, n).astype(np.float32)
+    ew = np.ones(n, np.float32)
+    ctx.points_upload(x, y, w)
+    ctx.set_bandwidths(hs)
+    ctx.set_ec_weights(ew)
+    g = M.grid(130, 140, 0.0, 0.0, 1.0)
+    mask = np.ones((140, 130), np.float32)
+    mask[60:64, :] = -9999.0
+    a = ctx.KernelDensityEstimation_tile(g, mask)
+    b = ctx.KernelDensityEstimation_tile(g, mask)
+    assert np.array_equal(a, b)
+    c = ctx.KernelDensityEstimation_raster(130, 140, [0.0, 1.0, 0.0, 140.0, 0.0, -1.0], -9999.0, 4096, 4096, mask)
+    assert np.array_equal(a, c)
+    og = orc.grid(130, 140, 0.0, 0.0, 1.0)
+    _, o64, _ = orc.density_tile(x, y, w, ew, hs, og, mask)
+    valid = mask != -9999.0
+    assert np.all(a[~valid] == -9999.0)
+    pos = valid & (o64 > 0)
+    assert np.max(np.abs(a[pos] - o64[pos]) / o64[pos]) <= REL
+    ctx.density_stats(True)
+    ctx.KernelDensityEstimation_tile(g, mask)
+    st = ctx.density_stats(False)
+    # the same job without the work list (raster order, no split): every
+    # candidate is classified exactly once either way
+    import os
+    os.environ["MKDE_DENSITY_ORDER"] = "0"
+    try:
+        c2 = M.Context(0)
+    finally:
+        del os.environ["MKDE_DENSITY_ORDER"]
+    c2.points_upload(x, y, w)
+    c2.set_bandwidths(hs)
+    c2.set_ec_weights(ew)
+    c2.density_stats(True)
+    d = c2.KernelDensityEstimation_tile(g, mask)
+    st2 = c2.density_stats(False)
+    c2.close()
+    assert st["candidates"] == st2["candidates"] and st["full_blocks"] == st2["full_blocks"]
+    assert np.max(np.abs(d[pos] - a[pos]) / a[pos]) <= 1e-6
