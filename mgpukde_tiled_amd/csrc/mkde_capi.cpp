@@ -43,6 +43,9 @@ hipError_t range_fill(hipStream_t, const PointIndex &, const int64_t *, int32_t 
 hipError_t valid_cells(hipStream_t, int64_t, const float *, float, uint8_t *);
 hipError_t boundary(hipStream_t, const GridDesc &, const uint8_t *, uint8_t *, float2 *, unsigned long long *);
 hipError_t dist_boundary(hipStream_t, int64_t, const float *, const float *, const float2 *, int64_t, double *);
+hipError_t bt_build(hipStream_t, const GridDesc &, const uint8_t *, uint4 *, float4 *, const BndTree &);
+hipError_t dist_tree(hipStream_t, int64_t, const float *, const float *, const int *, const GridDesc &,
+                     const BndTree &, double *);
 hipError_t ec_active(hipStream_t, int64_t, const double *, const float *, float, float *, int *, int *);
 hipError_t ec(hipStream_t, int, const GridDesc &, const MaskRuns &, const float *, const float *, const float *, float,
               const int *, float *, unsigned long long *);
@@ -173,6 +176,11 @@ struct mkde_ctx {
     DevBuf<float2> blist;
     DevBuf<unsigned long long> bcount;
     int64_t nboundary = -1;
+    // boundary hierarchy (nearest-boundary search)
+    DevBuf<uint4> bt_bits;
+    DevBuf<float4> bt_box;
+    BndTree bt{};
+    bool bt_ok = false;
 
     // tile
     bool tile_set = false;
@@ -562,6 +570,7 @@ extern "C" int mkde_mask_upload(mkde_ctx *c, const mkde_grid *g, const float *va
     HIPCHK(mkde_dev::valid_cells(c->stream, nc, c->mask.p, g->nodata, c->valid.p));
     c->mask_set = true;
     c->nboundary = -1;
+    c->bt_ok = false;
     c->dist_set = false;
     if (c->tile_is_ec_mask) c->tile_set = c->tile_is_ec_mask = false;
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -599,11 +608,47 @@ extern "C" int mkde_mark_boundary(mkde_ctx *c, uint8_t *flags, int64_t *n_bounda
     return MKDE_OK;
 }
 
+// Hierarchy over the boundary flags of the EC mask: 16x16-cell leaves up to
+// one root.  Rasters wider or taller than 16384 leaves use the brute-force
+// kernel instead.
+static int ensure_btree(mkde_ctx *c)
+{
+    if (c->bt_ok) return MKDE_OK;
+    BndTree &T = c->bt;
+    T = BndTree{};
+    int nx = (int)((c->mg.ncols + MKDE_BT_S - 1) / MKDE_BT_S), ny = (int)((c->mg.nrows + MKDE_BT_S - 1) / MKDE_BT_S);
+    if (nx > 16384 || ny > 16384) return MKDE_OK;
+    int64_t total = 0;
+    for (;;) {
+        T.nx[T.nlev] = nx;
+        T.ny[T.nlev] = ny;
+        T.off[T.nlev] = total;
+        total += (int64_t)nx * ny;
+        T.nlev++;
+        if (nx == 1 && ny == 1) break;
+        nx = (nx + 1) / 2;
+        ny = (ny + 1) / 2;
+    }
+    CHK(c->bt_bits.ensure(2 * (size_t)T.nx[0] * T.ny[0]));
+    CHK(c->bt_box.ensure(total));
+    HIPCHK(mkde_dev::bt_build(c->stream, c->mg, c->bflags.p, c->bt_bits.p, c->bt_box.p, T));
+    T.bits = c->bt_bits.p;
+    T.box = c->bt_box.p;
+    c->bt_ok = true;
+    return MKDE_OK;
+}
+
 static int ensure_dist(mkde_ctx *c)
 {
     if (c->dist_set) return MKDE_OK;
     CHK(ensure_boundary(c));
-    HIPCHK(mkde_dev::dist_boundary(c->stream, c->n, c->x.p, c->y.p, c->blist.p, c->nboundary, c->dist.p));
+    CHK(ensure_btree(c));
+    if (c->bt.nlev > 0) {
+        CHK(prep(c));
+        HIPCHK(mkde_dev::dist_tree(c->stream, c->n, c->xs.p, c->ys.p, c->perm.p, c->mg, c->bt, c->dist.p));
+    } else {
+        HIPCHK(mkde_dev::dist_boundary(c->stream, c->n, c->x.p, c->y.p, c->blist.p, c->nboundary, c->dist.p));
+    }
     c->dist_set = true;
     return MKDE_OK;
 }
@@ -1282,6 +1327,7 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
         PhaseTimer pt(c->stream, &S.ms_boundary);
         if (o->recompute_boundary) {
             c->nboundary = -1;
+            c->bt_ok = false;
             c->dist_set = false;
         }
         CHK(ensure_dist(c));

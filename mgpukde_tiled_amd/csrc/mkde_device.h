@@ -73,6 +73,20 @@ struct DensityWork {
     unsigned *hist;                // 32
 };
 
+// Boundary-cell hierarchy for exact nearest-boundary distances: leaves of
+// 16 x 16 cells hold a 256-bit mask of their boundary cells; every level keeps
+// the bounding box {x0, y0, x1, y1} of the boundary cell centres below a node
+// (x0 > x1: empty); level l has nx[l] x ny[l] nodes starting at off[l].
+#define MKDE_BT_S 16
+#define MKDE_BT_MAXL 15
+struct BndTree {
+    const uint4 *bits;     // 2 per leaf (256 bits, row-major within the leaf)
+    const float4 *box;     // all levels
+    int nlev;
+    int nx[MKDE_BT_MAXL], ny[MKDE_BT_MAXL];
+    int64_t off[MKDE_BT_MAXL];
+};
+
 // Per-row runs of valid cells of a mask (CSR): runs of row r are
 // run_se[run_ptr[r] .. run_ptr[r+1]), each (start, end) inclusive, ascending.
 struct MaskRuns {

@@ -29,6 +29,7 @@
 //                float-float prefix sums); FP64 only for log-likelihood sums.
 #include <hip/hip_runtime.h>
 #include <float.h>
+#include <limits.h>
 #include <algorithm>
 #include <stdint.h>
 
@@ -1370,6 +1371,143 @@ __global__ __launch_bounds__(256) void k_dist_boundary(int64_t n, const float *x
     if (i < n) dist[i] = m;
 }
 
+// ---- boundary hierarchy and nearest-boundary search
+//
+// k_bt_leaf: one thread per 16x16 leaf: bit mask of its boundary cells and
+// the bounding box of their centres (cell_x is non-decreasing in the column,
+// cell_y non-increasing in the row, so the extremes are at the extreme
+// indices).  k_bt_up: parents' boxes as unions of up to 4 children.
+__global__ void k_bt_leaf(GridDesc g, const uint8_t *flags, int nx, int ny, uint4 *bits, float4 *box)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nx * ny) return;
+    const int bx = (int)(t % nx), by = (int)(t / nx);
+    const int64_t c0 = (int64_t)bx * MKDE_BT_S, r0 = (int64_t)by * MKDE_BT_S;
+    unsigned m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int cmin = INT_MAX, cmax = -1, rmin = INT_MAX, rmax = -1;
+    for (int r = 0; r < MKDE_BT_S; ++r) {
+        if (r0 + r >= g.nrows) break;
+        const uint8_t *row = flags + (r0 + r) * g.ncols + c0;
+        for (int c = 0; c < MKDE_BT_S; ++c) {
+            if (c0 + c >= g.ncols) break;
+            if (row[c]) {
+                const int k = r * MKDE_BT_S + c;
+                m[k >> 5] |= 1u << (k & 31);
+                cmin = min(cmin, c);
+                cmax = max(cmax, c);
+                rmin = min(rmin, r);
+                rmax = max(rmax, r);
+            }
+        }
+    }
+    bits[2 * t] = make_uint4(m[0], m[1], m[2], m[3]);
+    bits[2 * t + 1] = make_uint4(m[4], m[5], m[6], m[7]);
+    float4 b = make_float4(FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX);
+    if (cmax >= 0) {
+        const double nrm05 = (double)g.nrows - 0.5;
+        b.x = cell_x(g.xll, g.cs, c0 + cmin);
+        b.z = cell_x(g.xll, g.cs, c0 + cmax);
+        b.y = cell_y(g.yll, g.cs, nrm05, r0 + rmax);
+        b.w = cell_y(g.yll, g.cs, nrm05, r0 + rmin);
+    }
+    box[t] = b;
+}
+
+__global__ void k_bt_up(int cnx, int cny, const float4 *child, int nx, int ny, float4 *parent)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nx * ny) return;
+    const int px = (int)(t % nx), py = (int)(t / nx);
+    float4 b = make_float4(FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX);
+    for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx) {
+            const int cx = 2 * px + dx, cy = 2 * py + dy;
+            if (cx >= cnx || cy >= cny) continue;
+            const float4 c = child[(int64_t)cy * cnx + cx];
+            if (c.x > c.z) continue;
+            b.x = fminf(b.x, c.x);
+            b.y = fminf(b.y, c.y);
+            b.z = fmaxf(b.z, c.z);
+            b.w = fmaxf(b.w, c.w);
+        }
+    parent[t] = b;
+}
+
+// Lower bound of fl64(dx^2 + dy^2) over the cells of a node (gap1 is an exact
+// lower bound of |fl32(u - v)| over an interval; both squares are exact).
+__device__ __forceinline__ double node_lb(float4 b, float px, float py)
+{
+    return b.x > b.z ? DBL_MAX : sq2(gap1(px, b.x, b.z), gap1(py, b.y, b.w));
+}
+
+#define BT_STACK 48
+#define BT_WG 128
+
+// CalcDist2Boundary by depth-first search, nearest child first, pruning
+// nodes whose lower bound is not below the best squared distance so far:
+// the result is the brute-force minimum over all boundary cells, exactly.
+// Points are taken in bin order (neighbouring lanes search alike) and the
+// result is scattered to the caller's order.
+__global__ __launch_bounds__(BT_WG) void k_dist_tree(int64_t n, const float *xs, const float *ys, const int *perm,
+                                                     GridDesc g, BndTree T, double *dist)
+{
+    __shared__ unsigned s_stack[BT_STACK * BT_WG];
+    const int tid = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * BT_WG + tid;
+    if (i >= n) return;
+    const float px = xs[i], py = ys[i];
+    const double nrm05 = (double)g.nrows - 0.5;
+    double best = DBL_MAX;
+    int sp = 0;
+    const int top = T.nlev - 1;
+    s_stack[(sp++) * BT_WG + tid] = (unsigned)top << 28;
+    while (sp > 0) {
+        const unsigned e = s_stack[(--sp) * BT_WG + tid];
+        const int lev = (int)(e >> 28), iy = (int)((e >> 14) & 0x3fff), ix = (int)(e & 0x3fff);
+        const int64_t node = T.off[lev] + (int64_t)iy * T.nx[lev] + ix;
+        if (node_lb(T.box[node], px, py) >= best) continue;
+        if (lev == 0) {
+            const uint4 m0 = T.bits[2 * node], m1 = T.bits[2 * node + 1];
+            const unsigned m[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+            const int64_t c0 = (int64_t)ix * MKDE_BT_S, r0 = (int64_t)iy * MKDE_BT_S;
+            for (int w = 0; w < 8; ++w) {
+                unsigned v = m[w];
+                while (v) {
+                    const int k = w * 32 + __ffs(v) - 1;
+                    v &= v - 1;
+                    const float cx = cell_x(g.xll, g.cs, c0 + (k & (MKDE_BT_S - 1)));
+                    const float cy = cell_y(g.yll, g.cs, nrm05, r0 + (k >> 4));
+                    const double d2 = sq2(cx - px, cy - py);
+                    best = d2 < best ? d2 : best;
+                }
+            }
+            continue;
+        }
+        // children sorted by lower bound, pushed farthest first
+        const int cl = lev - 1;
+        unsigned ce[4];
+        double cb[4];
+        int nc = 0;
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int cx = 2 * ix + dx, cy = 2 * iy + dy;
+                if (cx >= T.nx[cl] || cy >= T.ny[cl]) continue;
+                const double lb = node_lb(T.box[T.off[cl] + (int64_t)cy * T.nx[cl] + cx], px, py);
+                if (lb >= best) continue;
+                int k = nc++;
+                while (k > 0 && cb[k - 1] < lb) {   // descending lower bounds
+                    cb[k] = cb[k - 1];
+                    ce[k] = ce[k - 1];
+                    --k;
+                }
+                cb[k] = lb;
+                ce[k] = ((unsigned)cl << 28) | ((unsigned)cy << 14) | (unsigned)cx;
+            }
+        for (int k = 0; k < nc; ++k) s_stack[(sp++) * BT_WG + tid] = ce[k];
+    }
+    dist[perm[i]] = best;
+}
+
 // ===================================================== edge correction ====
 //
 // EdgeCorrectionWeightsExact with O(R) work per point instead of O(R^2).
@@ -1858,6 +1996,25 @@ hipError_t boundary(hipStream_t s, const GridDesc &g, const uint8_t *valid, uint
 {
     int64_t nc = g.ncols * g.nrows;
     hipLaunchKernelGGL(k_boundary, dim3(nblk(nc, 256)), dim3(256), 0, s, g, valid, flags, blist, bcount);
+    return hipGetLastError();
+}
+
+hipError_t bt_build(hipStream_t s, const GridDesc &g, const uint8_t *flags, uint4 *bits, float4 *box, const BndTree &T)
+{
+    const int64_t n0 = (int64_t)T.nx[0] * T.ny[0];
+    hipLaunchKernelGGL(k_bt_leaf, dim3(nblk(n0, 256)), dim3(256), 0, s, g, flags, T.nx[0], T.ny[0], bits, box);
+    for (int l = 1; l < T.nlev; ++l) {
+        const int64_t nl = (int64_t)T.nx[l] * T.ny[l];
+        hipLaunchKernelGGL(k_bt_up, dim3(nblk(nl, 256)), dim3(256), 0, s, T.nx[l - 1], T.ny[l - 1], box + T.off[l - 1],
+                           T.nx[l], T.ny[l], box + T.off[l]);
+    }
+    return hipGetLastError();
+}
+
+hipError_t dist_tree(hipStream_t s, int64_t n, const float *xs, const float *ys, const int *perm, const GridDesc &g,
+                     const BndTree &T, double *dist)
+{
+    if (n > 0) hipLaunchKernelGGL(k_dist_tree, dim3(nblk(n, BT_WG)), dim3(BT_WG), 0, s, n, xs, ys, perm, g, T, dist);
     return hipGetLastError();
 }
 

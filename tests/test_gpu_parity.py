@@ -57,6 +57,43 @@ def test_boundary_flags_and_distances(ctx, orc, scene):
     assert np.array_equal(d, od), "boundary distances must be bit-exact"
 
 
+@pytest.mark.parametrize("shape", ["edges_only", "scattered"])
+def test_boundary_distances_large_raster(ctx, orc, shape):
+    # ragged raster (odd leaf counts at every level), non-integer origin and
+    # cellsize; points deep inside (edges only) and among scattered holes;
+    # distances bit-exact against a brute force over all boundary cells
+    import mgpukde_tiled_amd as M
+
+    nc, nr, cs = 1037, 781, 0.37
+    g = orc.grid(nc, nr, 12.5, -7.25, cs)
+    mask = np.ones((nr, nc), np.float32)
+    r = np.random.default_rng(11)
+    if shape == "scattered":
+        mask[r.random((nr, nc)) < 0.002] = -9999.0
+        mask[300:340, 500:900] = -9999.0
+    x = (12.5 + r.uniform(-5, nc * cs + 5, 4000)).astype(np.float32)
+    y = (-7.25 + r.uniform(-5, nr * cs + 5, 4000)).astype(np.float32)
+    ctx.points_upload(x, y, None)
+    ctx.mask_upload(mgrid(M, g), mask)
+    flags, nb = ctx.MarkBoundary()
+    oflags, onb = orc.mark_boundary(g, mask)
+    assert nb == onb and np.array_equal(flags, oflags)
+    d = ctx.CalcDist2Boundary()
+    rr, cc = np.nonzero(oflags.reshape(nr, nc))
+    bx = np.array([orc.col_x(g, int(c)) for c in np.unique(cc)], np.float32)
+    bxm = dict(zip(np.unique(cc), bx))
+    by = np.array([orc.row_y(g, int(q)) for q in np.unique(rr)], np.float32)
+    bym = dict(zip(np.unique(rr), by))
+    cxs = np.array([bxm[c] for c in cc], np.float32)
+    cys = np.array([bym[q] for q in rr], np.float32)
+    ref = np.empty(len(x))
+    for k in range(0, len(x), 256):
+        dx = (cxs[None, :] - x[k:k + 256, None]).astype(np.float64)
+        dy = (cys[None, :] - y[k:k + 256, None]).astype(np.float64)
+        ref[k:k + 256] = (dx * dx + dy * dy).min(axis=1)
+    assert np.array_equal(d, ref)
+
+
 @pytest.mark.parametrize("h", [6.0, 17.5])
 def test_ec_weights_fixed(ctx, orc, scene, h):
     import mgpukde_tiled_amd as M
