@@ -480,7 +480,11 @@ def bench_serve(args):
            "sequential_run_ms": {"p50": float(np.percentile(seq_run, 50)), "p99": float(np.percentile(seq_run, 99))},
            "identical_results": same, "peak_committed_bytes": info["peak_committed"], "budget_bytes": info["budget"],
            "mix": {"fixed": sum(j["hoption"] == 0 for j in jobs), "adaptive": sum(j["hoption"] == 2 for j in jobs),
-                   "search": sum(j["hoption"] == 3 for j in jobs)}}
+                   "search": sum(j["hoption"] == 3 for j in jobs)},
+           "sequential_ms_by_kind": {k: float(sum(r[0]["run_ms"] for j, r in zip(jobs, seq) if j["hoption"] == h))
+                                     for k, h in (("fixed", 0), ("adaptive", 2), ("search", 3))},
+           "slowest_sequential": [{"points": jobs[i]["n"], "side": jobs[i]["side"], "hoption": jobs[i]["hoption"],
+                                   "run_ms": seq[i][0]["run_ms"]} for i in np.argsort(-seq_run)[:6]]}
     print(json.dumps(out), flush=True)
     return out
 

@@ -36,8 +36,9 @@ hipError_t density_tile(hipStream_t, const GridDesc &, const float *, float *, c
                         const Tiling &);
 hipError_t density_merge(hipStream_t, const GridDesc &, const Tiling &, const float *, int, const int4 *,
                          const double *, float *);
-hipError_t point_density(hipStream_t, const PointIndex &, int, float *, float);
-hipError_t loo_multi(hipStream_t, const PointIndex &, const float *, const double *, const double *, int, float *);
+hipError_t point_density(hipStream_t, const PointIndex &, int, float *, float, unsigned *, unsigned *, int *);
+hipError_t loo_multi(hipStream_t, const PointIndex &, const float *, const double *, const double *, int, float *,
+                     unsigned *, unsigned *, int *);
 hipError_t range_count(hipStream_t, const PointIndex &, int64_t *);
 hipError_t range_fill(hipStream_t, const PointIndex &, const int64_t *, int32_t *);
 hipError_t valid_cells(hipStream_t, int64_t, const float *, float, uint8_t *);
@@ -56,7 +57,7 @@ hipError_t sumlog(hipStream_t, int64_t, int, const float *, int, double *, int, 
 hipError_t abramson(hipStream_t, int64_t, const int *, const float *, const double *, float, float, float *);
 hipError_t unsort_f32(hipStream_t, int64_t, const int *, const float *, float *);
 hipError_t sort_f32(hipStream_t, int64_t, const int *, const float *, float *);
-hipError_t coef_k(hipStream_t, int64_t, const int *, const float *, const float *, double, float *);
+hipError_t coef_k(hipStream_t, int64_t, const int *, const float *, const float *, double, float *, int);
 hipError_t fill_f32(hipStream_t, int64_t, float, float *);
 }  // namespace mkde_dev
 
@@ -199,6 +200,9 @@ struct mkde_ctx {
     DevBuf<float> coefk, ewk, den1k;
     DevBuf<double> ak, tk;
     DevBuf<float> hsave, ewsave;   // bandwidths / EC weights saved across a likelihood scan
+    // heaviest-first order of the 64-query waves of the point kernels
+    DevBuf<unsigned> pw_cost, pw_hist;
+    DevBuf<int> pw_order;
     // density kernel counters (mkde_density_stats)
     bool dt_stats_on = false;
     DevBuf<unsigned long long> dt_stats;
@@ -510,12 +514,12 @@ extern "C" int mkde_rule_of_thumb(mkde_ctx *c, float *h0)
 
 // ----------------------------------------------------------------- mask --
 
-static void build_runs(const std::vector<float> &v, int64_t ncols, int64_t nrows, float nodata, RunsHost &R)
+static void build_runs(const float *v, int64_t ncols, int64_t nrows, float nodata, RunsHost &R)
 {
     R.ptr.assign(nrows + 1, 0);
     R.se.clear();
     for (int64_t r = 0; r < nrows; ++r) {
-        const float *row = v.data() + r * ncols;
+        const float *row = v + r * ncols;
         int64_t c = 0;
         while (c < ncols) {
             while (c < ncols && row[c] == nodata) ++c;
@@ -551,9 +555,8 @@ extern "C" int mkde_mask_upload(mkde_ctx *c, const mkde_grid *g, const float *va
     CHK(set_device(c));
     const int64_t nc = g->ncols * g->nrows;
     try {
-        std::vector<float> v(values, values + nc);
         RunsHost R;
-        build_runs(v, g->ncols, g->nrows, g->nodata, R);
+        build_runs(values, g->ncols, g->nrows, g->nodata, R);
         c->mg = to_desc(g);
         CHK(c->mask.ensure(nc));
         CHK(c->valid.ensure(nc));
@@ -810,12 +813,23 @@ extern "C" int mkde_range_candidates(mkde_ctx *c, int64_t *offsets, int32_t *idx
 
 // ------------------------------------------------- densities at points --
 
+static int ensure_wave_order(mkde_ctx *c)
+{
+    const int64_t nw = (c->n + 63) / 64;
+    CHK(c->pw_cost.ensure(nw));
+    CHK(c->pw_hist.ensure(32));
+    CHK(c->pw_order.ensure(nw));
+    return MKDE_OK;
+}
+
 static int density_points_sorted(mkde_ctx *c, int loo, float *den_sorted)
 {
     CHK(prep(c));
     PointIndex I = index_of(c);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(mkde_dev::point_density(c->stream, I, loo, den_sorted, c->h_fixed));
+    CHK(ensure_wave_order(c));
+    HIPCHK(mkde_dev::point_density(c->stream, I, loo, den_sorted, c->h_fixed, c->pw_cost.p, c->pw_hist.p,
+                                   c->pw_order.p));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     return MKDE_OK;
 }
@@ -869,7 +883,7 @@ extern "C" int mkde_loo_loglik(mkde_ctx *c, int K, const float *h_cand, int ec_o
     CHK(prep(c));
     PointIndex I = index_of(c);
     const int G = std::min(K, MKDE_LM_K);
-    CHK(c->coefk.ensure((size_t)G * n));
+    CHK(c->coefk.ensure((size_t)MKDE_LM_K * n));   // [n][MKDE_LM_K]: a point's candidate slots contiguous
     CHK(c->den1k.ensure((size_t)G * n));
     CHK(c->ewk.ensure(n));
     CHK(c->ak.ensure(G));
@@ -886,12 +900,14 @@ extern "C" int mkde_loo_loglik(mkde_ctx *c, int K, const float *h_cand, int ec_o
             t[j] = 9.0 * h2;
             if (ec_on) CHK(run_ec(c, nullptr, hk, c->ewk.p, nullptr));
             else HIPCHK(mkde_dev::fill_f32(c->stream, n, 1.0f, c->ewk.p));
-            HIPCHK(mkde_dev::coef_k(c->stream, n, c->perm.p, c->w.p, c->ewk.p, h2, c->coefk.p + (size_t)j * n));
+            HIPCHK(mkde_dev::coef_k(c->stream, n, c->perm.p, c->w.p, c->ewk.p, h2, c->coefk.p + j, MKDE_LM_K));
         }
         CHK(copy_h2d(c, c->ak.p, a.data(), kg * sizeof(double)));
         CHK(copy_h2d(c, c->tk.p, t.data(), kg * sizeof(double)));
         HIPCHK(hipEventRecord(c->ev0, c->stream));
-        HIPCHK(mkde_dev::loo_multi(c->stream, I, c->coefk.p, c->ak.p, c->tk.p, kg, c->den1k.p));
+        CHK(ensure_wave_order(c));
+        HIPCHK(mkde_dev::loo_multi(c->stream, I, c->coefk.p, c->ak.p, c->tk.p, kg, c->den1k.p, c->pw_cost.p,
+                                   c->pw_hist.p, c->pw_order.p));
         HIPCHK(hipEventRecord(c->ev1, c->stream));
         CHK(sum_logs(c, c->den1k.p, kg, 1, c->dsum.p));
         std::vector<double> part(kg);
@@ -1212,12 +1228,10 @@ extern "C" int mkde_count_density_pairs(mkde_ctx *c, const mkde_grid *tile, cons
     CHK(need_h(c));
     CHK(check_grid(tile));
     if (!n_pairs) return fail(MKDE_E_ARG, "n_pairs is NULL");
-    const int64_t nc = tile->ncols * tile->nrows;
     RunsHost R;
     try {
         if (mask) {
-            std::vector<float> v(mask, mask + nc);
-            build_runs(v, tile->ncols, tile->nrows, tile->nodata, R);
+            build_runs(mask, tile->ncols, tile->nrows, tile->nodata, R);
         } else {
             R.ptr.resize(tile->nrows + 1);
             R.se.resize(tile->nrows);

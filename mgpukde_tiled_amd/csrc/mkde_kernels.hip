@@ -7,10 +7,10 @@
 //   k_ec              per-point edge-correction integral, O(R) per point
 //                     (replaces CalcEdgeCorrectionWeights kde_kernel_kdtr.cu:117-265
 //                     and EdgeCorrectionWeightsExact tiled:2694-2807)
-//   k_point_density   density at points over the cell-binned index, pilot and
+//   k_point_density_w density at points over the cell-binned index, pilot and
 //                     leave-one-out forms (replaces DensityAtPointsKdtr
 //                     kde_kernel_kdtr.cu:681-953 and the CPU scatter tiled:3312-3347)
-//   k_loo_multi       leave-one-out densities of up to 16 fixed bandwidths in one
+//   k_loo_multi_w     leave-one-out densities of up to 16 fixed bandwidths in one
 //                     pass over point pairs (batched LogLikelihood, tiled:3214-3368)
 //   k_boundary, k_dist_boundary   MarkBoundary / CalcDist2Boundary
 //                     (kde_kernel_kdtr.cu:1209-1353, tiled:3993-4131)
@@ -226,6 +226,54 @@ __global__ void k_bin_stats(int nbins, const int *bin_start, const PtRec *pts, B
     box[b] = BinInfo{x0, y0, x1, y1};
     thr[b] = t;
 }
+
+// ---- wave-uniform row values by DPP broadcast
+//
+// Ey of a wave's 16 rows lives in one VGPR: lane l holds row (l & 15), so
+// every 16-lane row of the wave carries all 16 values and DPP row_newbcast:j
+// hands row j's value to all 64 lanes as the FMA's first operand.  This
+// replaces a 64-lane LDS broadcast read per value.  Leading s_nop 1 gives the
+// two wait states a DPP read needs after a VALU write of its operands.
+#define MKDE_FMA_BC(J) "v_fmac_f32_dpp %" #J ", %16, %17 row_newbcast:" #J " row_mask:0xf bank_mask:0xf\n"
+
+// acc[j] += Ey[j] * ex, j = 0..15
+__device__ __forceinline__ void fma16_bcast(float (&a)[16], float eyv, float ex)
+{
+    asm("s_nop 1\n" MKDE_FMA_BC(0) MKDE_FMA_BC(1) MKDE_FMA_BC(2) MKDE_FMA_BC(3) MKDE_FMA_BC(4) MKDE_FMA_BC(5)
+            MKDE_FMA_BC(6) MKDE_FMA_BC(7) MKDE_FMA_BC(8) MKDE_FMA_BC(9) MKDE_FMA_BC(10) MKDE_FMA_BC(11)
+                MKDE_FMA_BC(12) MKDE_FMA_BC(13) MKDE_FMA_BC(14) MKDE_FMA_BC(15)
+        : "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3]), "+v"(a[4]), "+v"(a[5]), "+v"(a[6]), "+v"(a[7]), "+v"(a[8]),
+          "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+        : "v"(eyv), "v"(ex));
+}
+
+// Rows R..R+3 of a PARTIAL sub-block: m_i = fl64(dx2 + dy2_i) < thr ? ex : 0
+// (the exact cut-off), then acc[R+i] += Ey[R+i] * m_i.  The four masks are
+// formed before the four DPP FMAs, so each m_i is at least 2 VALU
+// instructions old when it is read.
+#define MKDE_PART_ROW(I) \
+    "v_add_f64 %[t], %[dx2], %[d" #I "]\n"   \
+    "v_cmp_lt_f64 vcc, %[t], %[thr]\n"        \
+    "v_cndmask_b32 %[m" #I "], 0, %[ex], vcc\n"
+#define MKDE_PART_FMA(I, J) "v_fmac_f32_dpp %[a" #I "], %[ey], %[m" #I "] row_newbcast:" #J " row_mask:0xf bank_mask:0xf\n"
+#define MKDE_PART4(R0, R1, R2, R3)                                                                                    \
+    __device__ __forceinline__ void part4_##R0(float &a0, float &a1, float &a2, float &a3, double dx2, double d0,      \
+                                               double d1, double d2, double d3, double thr, float ex, float eyv)      \
+    {                                                                                                                 \
+        float m0, m1, m2, m3;                                                                                         \
+        double t;                                                                                                     \
+        asm(MKDE_PART_ROW(0) MKDE_PART_ROW(1) MKDE_PART_ROW(2) MKDE_PART_ROW(3) MKDE_PART_FMA(0, R0)                 \
+                MKDE_PART_FMA(1, R1) MKDE_PART_FMA(2, R2) MKDE_PART_FMA(3, R3)                                        \
+            : [a0] "+v"(a0), [a1] "+v"(a1), [a2] "+v"(a2), [a3] "+v"(a3), [m0] "=&v"(m0), [m1] "=&v"(m1),             \
+              [m2] "=&v"(m2), [m3] "=&v"(m3), [t] "=&v"(t)                                                            \
+            : [dx2] "v"(dx2), [d0] "v"(d0), [d1] "v"(d1), [d2] "v"(d2), [d3] "v"(d3), [thr] "v"(thr), [ex] "v"(ex),   \
+              [ey] "v"(eyv)                                                                                           \
+            : "vcc");                                                                                                 \
+    }
+MKDE_PART4(0, 1, 2, 3)
+MKDE_PART4(4, 5, 6, 7)
+MKDE_PART4(8, 9, 10, 11)
+MKDE_PART4(12, 13, 14, 15)
 
 // ======================================================== density raster ====
 //
@@ -466,21 +514,26 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
             }
         }
         __syncthreads();
-        // 2a. block-FULL points: one FMA per cell, no tests
+        // 2a. block-FULL points: one FMA per cell, no tests.  Ex of this
+        //     lane's column from LDS, Ey of the wave's 16 rows as one VGPR
+        //     (lane l: row l & 15) broadcast by DPP.
         const int nfull = s_nfull;
-#pragma unroll 2
-        for (int p = 0; p < nfull; ++p) {
-            const float ex = s_ex[p * 64 + lane];
-            const float4 *eyp = (const float4 *)&s_ey[p * DT_EYP + wave * 16];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float4 v = eyp[k];
-                acc[4 * k] = __builtin_fmaf(ex, v.x, acc[4 * k]);
-                acc[4 * k + 1] = __builtin_fmaf(ex, v.y, acc[4 * k + 1]);
-                acc[4 * k + 2] = __builtin_fmaf(ex, v.z, acc[4 * k + 2]);
-                acc[4 * k + 3] = __builtin_fmaf(ex, v.w, acc[4 * k + 3]);
+        const int eyo = wave * 16 + (lane & 15);
+        {
+            // two points per step: the second pair of loads is in flight
+            // while the first point's 16 FMAs run
+            int p = 0;
+            for (; p + 1 < nfull; p += 2) {
+                const float ey0 = s_ey[p * DT_EYP + eyo], ex0 = s_ex[p * 64 + lane];
+                const float ey1 = s_ey[(p + 1) * DT_EYP + eyo], ex1 = s_ex[(p + 1) * 64 + lane];
+                fma16_bcast(acc, ey0, ex0);
+                fma16_bcast(acc, ey1, ex1);
+                if (STATS && lane == 0) st_wf += 2;
             }
-            if (STATS && lane == 0) ++st_wf;
+            if (p < nfull) {
+                fma16_bcast(acc, s_ey[p * DT_EYP + eyo], s_ex[p * 64 + lane]);
+                if (STATS && lane == 0) ++st_wf;
+            }
         }
         // 2b. block-PARTIAL points: per-wave class
         for (int p = nfull; p < cnt; ++p) {
@@ -490,32 +543,25 @@ __global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
                 continue;
             }
             const float ex = s_ex[p * 64 + lane];
-            const float4 *eyp = (const float4 *)&s_ey[p * DT_EYP + wave * 16];
-            float ey[16];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                float4 v = eyp[k];
-                ey[4 * k] = v.x;
-                ey[4 * k + 1] = v.y;
-                ey[4 * k + 2] = v.z;
-                ey[4 * k + 3] = v.w;
-            }
+            const float eyv = s_ey[p * DT_EYP + eyo];
             if (cls == 1) {
-#pragma unroll
-                for (int j = 0; j < 16; ++j) acc[j] = __builtin_fmaf(ex, ey[j], acc[j]);
+                fma16_bcast(acc, eyv, ex);
                 if (STATS && lane == 0) ++st_wf;
             } else {
                 const double thr = s_thr[p];
                 const double dx2 = sqd(cx - s_px[p]);
                 const double2 *dyp = (const double2 *)&s_dy2[p * DT_DYP + wave * 16];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    double2 v = dyp[k];
-                    bool in0 = __dadd_rn(dx2, v.x) < thr;
-                    bool in1 = __dadd_rn(dx2, v.y) < thr;
-                    acc[2 * k] = __builtin_fmaf(in0 ? ex : 0.0f, ey[2 * k], acc[2 * k]);
-                    acc[2 * k + 1] = __builtin_fmaf(in1 ? ex : 0.0f, ey[2 * k + 1], acc[2 * k + 1]);
-                }
+                double2 u = dyp[0], v = dyp[1];
+                part4_0(acc[0], acc[1], acc[2], acc[3], dx2, u.x, u.y, v.x, v.y, thr, ex, eyv);
+                u = dyp[2];
+                v = dyp[3];
+                part4_4(acc[4], acc[5], acc[6], acc[7], dx2, u.x, u.y, v.x, v.y, thr, ex, eyv);
+                u = dyp[4];
+                v = dyp[5];
+                part4_8(acc[8], acc[9], acc[10], acc[11], dx2, u.x, u.y, v.x, v.y, thr, ex, eyv);
+                u = dyp[6];
+                v = dyp[7];
+                part4_12(acc[12], acc[13], acc[14], acc[15], dx2, u.x, u.y, v.x, v.y, thr, ex, eyv);
                 if (STATS && lane == 0) ++st_wp;
             }
         }
@@ -846,12 +892,12 @@ __global__ void k_scan_add(int n, unsigned *out, const unsigned *chunk_sums)
 
 // ==================================================== density at points ====
 //
-// One 256-thread workgroup owns 256 consecutive query points in bin order.
-// Source points of the candidate bins are streamed through LDS 256 at a time
-// (several bins per batch), and every thread tests and accumulates all of
-// them for its query.  Each source uses its own bandwidth (sample-point
-// estimator); FIXED specialises the common case of one bandwidth.  LOO
-// excludes the query itself (same index); duplicates at distance 0 are kept.
+// One wave owns 64 consecutive query points in bin order (lane = query) and
+// walks the candidate bins of its bounding box; the source records are
+// wave-uniform and come in by scalar loads.  Each source uses its own
+// bandwidth (sample-point estimator); FIXED specialises the common case of
+// one bandwidth.  LOO excludes the query itself (same index); duplicates at
+// distance 0 are kept.  Waves are launched heaviest first (k_wave_cost).
 //
 // Cut-off test: d2f = fl32(dx*dx + dy*dy) decides unless it lies within a
 // relative 2^-20 band around the threshold (its error is below 2^-22
@@ -863,9 +909,181 @@ struct PtsArgs {
     float *den;   // output, bin order
     int loo;
     double thr_fixed, a_fixed;   // FIXED kernels only
+    const int *order;            // nullable: waves heaviest first
 };
 
 #define PD_BAND (1.0 / 1048576.0)
+
+// Per-wave variant: each wave owns 64 consecutive queries in bin order and
+// walks the candidate bins of its own bounding box; a source's record is
+// wave-uniform, so it is read with scalar loads into SGPRs (no LDS, and the
+// VALU reads SGPR operands for free).  FIXED: FP32 screen with the FP64
+// re-check of a 32-source chunk when any lane is inside the 2^-20 band;
+// otherwise (per-source bandwidths) the exact FP64 test for every pair.
+__device__ __forceinline__ float wave_min(float v)
+{
+    for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, WAVE));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v)
+{
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, WAVE));
+    return v;
+}
+__device__ __forceinline__ float uni(float v) { return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v))); }
+
+// Work estimate of each 64-query wave of the point kernels: the points in
+// its candidate bins (same culling as the kernels: bins within the reach of
+// the wave's bounding box whose box gap is below the threshold -- lim_fixed
+// when > 0, else lim_dev[0] when given, else each bin's own threshold).
+// Waves then run heaviest first.
+__global__ __launch_bounds__(256) void k_wave_cost(PointIndex I, double lim_fixed, const double *lim_dev,
+                                                   unsigned *cost)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = (I.n + 63) / 64;
+    const int64_t wv = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= nw) return;
+    const int64_t q = wv * 64 + lane;
+    const bool qv = q < I.n;
+    float qx = 0.0f, qy = 0.0f;
+    if (qv) {
+        const PtRec p = I.pts[q];
+        qx = p.x;
+        qy = p.y;
+    }
+    const float mnx = wave_min(qv ? qx : FLT_MAX), mxx = wave_max(qv ? qx : -FLT_MAX);
+    const float mny = wave_min(qv ? qy : FLT_MAX), mxy = wave_max(qv ? qy : -FLT_MAX);
+    const bool uni_lim = lim_fixed > 0.0 || lim_dev;
+    const double lim = lim_fixed > 0.0 ? lim_fixed : (lim_dev ? *lim_dev : 0.0);
+    const double reach = uni_lim ? sqrt(fmax(lim, 0.0)) * 1.000001 + 1e-6 : *I.reach_p;
+    int bxa, bxb, bya, byb;
+    bin_range(I.bg, mnx, mxx, mny, mxy, reach, bxa, bxb, bya, byb);
+    const int nbx_r = bxb - bxa + 1;
+    const int nb_total = nbx_r * (byb - bya + 1);
+    unsigned acc = 0;
+    for (int k = lane; k < nb_total; k += 64) {
+        const int b = (bya + k / nbx_r) * I.bg.nbx + (bxa + k % nbx_r);
+        const int c = I.bin_start[b + 1] - I.bin_start[b];
+        if (c > 0 && box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < (uni_lim ? lim : I.bin_thr[b])) acc += (unsigned)c;
+    }
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, WAVE);
+    if (lane == 0) cost[wv] = acc;
+}
+
+__global__ void k_cost_hist(int n, const unsigned *cost, unsigned *hist)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&hist[cost_bucket(cost[i])], 1u);
+}
+
+__global__ void k_cost_order(int n, const unsigned *cost, unsigned *offs, int *order)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) order[atomicAdd(&offs[cost_bucket(cost[i])], 1u)] = i;
+}
+
+template <bool FIXED, bool LOO>
+__global__ __launch_bounds__(256) void k_point_density_w(PtsArgs A)
+{
+    const PointIndex &I = A.idx;
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = (I.n + 63) / 64;
+    int64_t wv = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= nw) return;   // whole wave
+    if (A.order) wv = A.order[wv];
+    const int64_t q0 = wv * 64;
+    const int64_t q = q0 + lane;
+    const bool qv = q < I.n;
+    float qx = 0.0f, qy = 0.0f;
+    if (qv) {
+        const PtRec p = I.pts[q];
+        qx = p.x;
+        qy = p.y;
+    }
+    const float mnx = uni(wave_min(qv ? qx : FLT_MAX)), mxx = uni(wave_max(qv ? qx : -FLT_MAX));
+    const float mny = uni(wave_min(qv ? qy : FLT_MAX)), mxy = uni(wave_max(qv ? qy : -FLT_MAX));
+    const double reach = FIXED ? sqrt(fmax(A.thr_fixed, 0.0)) * 1.000001 + 1e-6 : *I.reach_p;
+    int bxa, bxb, bya, byb;
+    bin_range(I.bg, mnx, mxx, mny, mxy, reach, bxa, bxb, bya, byb);
+    bxa = __builtin_amdgcn_readfirstlane(bxa);
+    bxb = __builtin_amdgcn_readfirstlane(bxb);
+    bya = __builtin_amdgcn_readfirstlane(bya);
+    byb = __builtin_amdgcn_readfirstlane(byb);
+
+    const float f_thr_lo = FIXED ? (float)(A.thr_fixed * (1.0 - PD_BAND)) : 0.0f;
+    const float f_thr_hi = FIXED ? fmaxf((float)(A.thr_fixed * (1.0 + PD_BAND)), FLT_MIN) : 0.0f;
+    const float f_a = FIXED ? (float)A.a_fixed : 0.0f;
+    float acc = 0.0f, sum = 0.0f, comp = 0.0f;
+    for (int by = bya; by <= byb; ++by) {
+        for (int bx = bxa; bx <= bxb; ++bx) {
+            const int b = by * I.bg.nbx + bx;
+            const int st = I.bin_start[b], en = I.bin_start[b + 1];
+            if (st == en) continue;
+            const double lim = FIXED ? A.thr_fixed : I.bin_thr[b];
+            if (!__builtin_amdgcn_readfirstlane((int)(box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < lim))) continue;
+            for (int c0 = st; c0 < en; c0 += 32) {
+                const int c1 = c0 + 32 < en ? c0 + 32 : en;
+                // four records are loaded before their arithmetic, so the
+                // scalar loads overlap
+                if (FIXED) {
+                    bool band = false;
+                    auto body = [&](const PtRec &p, int si) {
+                        const float dx = p.x - qx, dy = p.y - qy;
+                        const float d2f = __builtin_fmaf(dy, dy, dx * dx);
+                        bool in = d2f < f_thr_lo;
+                        band |= (d2f < f_thr_hi) != in;
+                        if (LOO && (int64_t)si == q) in = false;
+                        const float e = fast_exp2(d2f * f_a);
+                        acc = __builtin_fmaf(in ? e : 0.0f, p.coef, acc);
+                    };
+                    int si = c0;
+                    for (; si + 4 <= c1; si += 4) {
+                        const PtRec p0 = I.pts[si], p1 = I.pts[si + 1], p2 = I.pts[si + 2], p3 = I.pts[si + 3];
+                        body(p0, si);
+                        body(p1, si + 1);
+                        body(p2, si + 2);
+                        body(p3, si + 3);
+                    }
+                    for (; si < c1; ++si) body(I.pts[si], si);
+                    if (__any(band)) {   // rare: exact FP64 test for the chunk
+                        acc = 0.0f;
+                        for (int sj = c0; sj < c1; ++sj) {
+                            const PtRec p = I.pts[sj];
+                            const float dx = p.x - qx, dy = p.y - qy;
+                            const float d2f = __builtin_fmaf(dy, dy, dx * dx);
+                            bool in = sq2(dx, dy) < A.thr_fixed;
+                            if (LOO && (int64_t)sj == q) in = false;
+                            const float e = fast_exp2(d2f * f_a);
+                            acc = __builtin_fmaf(in ? e : 0.0f, p.coef, acc);
+                        }
+                    }
+                } else {
+                    auto body = [&](const PtRec &p, int si) {
+                        const float dx = p.x - qx, dy = p.y - qy;
+                        const float d2f = __builtin_fmaf(dy, dy, dx * dx);
+                        bool in = sq2(dx, dy) < p.thr;
+                        if (LOO && (int64_t)si == q) in = false;
+                        const float e = fast_exp2(d2f * (float)p.a);
+                        acc = __builtin_fmaf(in ? e : 0.0f, p.coef, acc);
+                    };
+                    int si = c0;
+                    for (; si + 4 <= c1; si += 4) {
+                        const PtRec p0 = I.pts[si], p1 = I.pts[si + 1], p2 = I.pts[si + 2], p3 = I.pts[si + 3];
+                        body(p0, si);
+                        body(p1, si + 1);
+                        body(p2, si + 2);
+                        body(p3, si + 3);
+                    }
+                    for (; si < c1; ++si) body(I.pts[si], si);
+                }
+                kahan_fold(acc, sum, comp);
+                acc = 0.0f;
+            }
+        }
+    }
+    if (qv) A.den[q] = sum;
+}
 
 // Exclusive scan of one int per thread across a 256-thread block.
 __device__ __forceinline__ int block_exscan256(int v, int *s_w, int *total)
@@ -886,166 +1104,7 @@ __device__ __forceinline__ int block_exscan256(int v, int *s_w, int *total)
     return off + inc - v;
 }
 
-// Bounding box of the valid queries of the block (all threads get it).
-__device__ __forceinline__ void block_bbox(bool qv, float qx, float qy, float (*s_red)[4], float &mnx, float &mxx,
-                                           float &mny, float &mxy)
-{
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    mnx = qv ? qx : FLT_MAX;
-    mxx = qv ? qx : -FLT_MAX;
-    mny = qv ? qy : FLT_MAX;
-    mxy = qv ? qy : -FLT_MAX;
-    for (int d = 32; d >= 1; d >>= 1) {
-        mnx = fminf(mnx, __shfl_xor(mnx, d, WAVE));
-        mxx = fmaxf(mxx, __shfl_xor(mxx, d, WAVE));
-        mny = fminf(mny, __shfl_xor(mny, d, WAVE));
-        mxy = fmaxf(mxy, __shfl_xor(mxy, d, WAVE));
-    }
-    if (lane == 0) {
-        s_red[wave][0] = mnx;
-        s_red[wave][1] = mxx;
-        s_red[wave][2] = mny;
-        s_red[wave][3] = mxy;
-    }
-    __syncthreads();
-    mnx = fminf(fminf(s_red[0][0], s_red[1][0]), fminf(s_red[2][0], s_red[3][0]));
-    mxx = fmaxf(fmaxf(s_red[0][1], s_red[1][1]), fmaxf(s_red[2][1], s_red[3][1]));
-    mny = fminf(fminf(s_red[0][2], s_red[1][2]), fminf(s_red[2][2], s_red[3][2]));
-    mxy = fmaxf(fmaxf(s_red[0][3], s_red[1][3]), fmaxf(s_red[2][3], s_red[3][3]));
-    __syncthreads();
-}
 
-template <bool FIXED, bool LOO>
-__global__ __launch_bounds__(256) void k_point_density(PtsArgs A)
-{
-    __shared__ int s_bins[256];
-    __shared__ int s_pref[257];
-    __shared__ int s_w[4];
-    __shared__ float4 s_s4[256];      // x, y, coef, a (float)
-    __shared__ float2 s_band[256];    // thr*(1-band), thr*(1+band) as float
-    __shared__ double s_thr[256];
-    __shared__ int s_idx[256];
-    __shared__ float s_red[4][4];
-
-    const PointIndex &I = A.idx;
-    const int tid = threadIdx.x;
-    const int64_t q = (int64_t)blockIdx.x * 256 + tid;
-    const bool qv = q < I.n;
-    float qx = 0.0f, qy = 0.0f;
-    if (qv) {
-        const PtRec p = I.pts[q];
-        qx = p.x;
-        qy = p.y;
-    }
-    float mnx, mxx, mny, mxy;
-    block_bbox(qv, qx, qy, s_red, mnx, mxx, mny, mxy);
-
-    const double reach = FIXED ? sqrt(fmax(A.thr_fixed, 0.0)) * 1.000001 + 1e-6 : *I.reach_p;
-    int bxa, bxb, bya, byb;
-    bin_range(I.bg, mnx, mxx, mny, mxy, reach, bxa, bxb, bya, byb);
-    const int nbx_r = bxb - bxa + 1;
-    const int nb_total = nbx_r * (byb - bya + 1);
-
-    const float f_thr_lo = FIXED ? (float)(A.thr_fixed * (1.0 - PD_BAND)) : 0.0f;
-    const float f_thr_hi = FIXED ? fmaxf((float)(A.thr_fixed * (1.0 + PD_BAND)), FLT_MIN) : 0.0f;
-    const float f_a = FIXED ? (float)A.a_fixed : 0.0f;
-
-    float acc = 0.0f, sum = 0.0f, comp = 0.0f;
-    for (int base = 0; base < nb_total; base += 256) {
-        const int kk = base + tid;
-        int cntb = 0;
-        int b = 0;
-        if (kk < nb_total) {
-            b = (bya + kk / nbx_r) * I.bg.nbx + (bxa + kk % nbx_r);
-            const int c = I.bin_start[b + 1] - I.bin_start[b];
-            if (c > 0) {
-                const double lim = FIXED ? A.thr_fixed : I.bin_thr[b];
-                if (box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < lim) cntb = c;
-            }
-        }
-        int nb;
-        const int slot = block_compact256(cntb > 0, s_w, &nb);
-        if (cntb > 0) s_bins[slot] = b;
-        int T;
-        // counts of the compacted bins, in slot order
-        __syncthreads();
-        const int mycnt = tid < nb ? I.bin_start[s_bins[tid] + 1] - I.bin_start[s_bins[tid]] : 0;
-        const int pre = block_exscan256(mycnt, s_w, &T);
-        if (tid < nb) s_pref[tid] = pre;
-        if (tid == 0) s_pref[nb] = T;
-        __syncthreads();
-        for (int g0 = 0; g0 < T; g0 += 256) {
-            const int gidx = g0 + tid;
-            const int cnt = (T - g0) < 256 ? (T - g0) : 256;
-            if (gidx < T) {
-                int lo = 0, hi = nb - 1;   // last k with s_pref[k] <= gidx
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_pref[mid] <= gidx) lo = mid;
-                    else hi = mid - 1;
-                }
-                const int si = I.bin_start[s_bins[lo]] + (gidx - s_pref[lo]);
-                const PtRec p = I.pts[si];
-                if (FIXED) {
-                    s_s4[tid] = make_float4(p.x, p.y, p.coef, 0.0f);
-                } else {
-                    s_s4[tid] = make_float4(p.x, p.y, p.coef, (float)p.a);
-                    s_band[tid] = make_float2((float)(p.thr * (1.0 - PD_BAND)),
-                                              fmaxf((float)(p.thr * (1.0 + PD_BAND)), FLT_MIN));
-                    s_thr[tid] = p.thr;
-                }
-                s_idx[tid] = si;
-            }
-            __syncthreads();
-            for (int s0 = 0; s0 < cnt; s0 += 32) {
-                const int s1 = (s0 + 32 < cnt) ? s0 + 32 : cnt;
-                // fast pass: FP32 cut-off, noting sources inside the guard band
-                bool band = false;
-#pragma unroll 4
-                for (int s = s0; s < s1; ++s) {
-                    const float4 src = s_s4[s];
-                    const float dx = src.x - qx, dy = src.y - qy;
-                    const float d2f = __builtin_fmaf(dy, dy, dx * dx);
-                    float tlo, thi, af;
-                    if (FIXED) {
-                        tlo = f_thr_lo;
-                        thi = f_thr_hi;
-                        af = f_a;
-                    } else {
-                        const float2 bd = s_band[s];
-                        tlo = bd.x;
-                        thi = bd.y;
-                        af = src.w;
-                    }
-                    bool in = d2f < tlo;
-                    band |= (d2f < thi) != in;
-                    if (LOO && (int64_t)s_idx[s] == q) in = false;
-                    const float e = fast_exp2(d2f * af);
-                    acc = __builtin_fmaf(in ? e : 0.0f, src.z, acc);
-                }
-                // rare: some lane of the wave met a source in the band -- redo
-                // the chunk with the exact FP64 test for the whole wave
-                if (__any(band)) {
-                    acc = 0.0f;
-                    for (int s = s0; s < s1; ++s) {
-                        const float4 src = s_s4[s];
-                        const float dx = src.x - qx, dy = src.y - qy;
-                        const float d2f = __builtin_fmaf(dy, dy, dx * dx);
-                        const double t = FIXED ? A.thr_fixed : s_thr[s];
-                        bool in = sq2(dx, dy) < t;
-                        if (LOO && (int64_t)s_idx[s] == q) in = false;
-                        const float e = fast_exp2(d2f * (FIXED ? f_a : src.w));
-                        acc = __builtin_fmaf(in ? e : 0.0f, src.z, acc);
-                    }
-                }
-                kahan_fold(acc, sum, comp);
-                acc = 0.0f;
-            }
-            __syncthreads();
-        }
-    }
-    if (qv) A.den[q] = sum;
-}
 
 // ============================================= batched LOO (fixed bandwidths) ==
 //
@@ -1060,30 +1119,33 @@ __global__ __launch_bounds__(256) void k_point_density(PtsArgs A)
 
 struct LooMultiArgs {
     PointIndex idx;          // positions + bins (thr of the largest candidate)
-    const float *coefk;      // [K][n] bin order
+    const float *coefk;      // [n][MKDE_LM_K] bin order (candidate slots of a point contiguous)
     const double *a_k;       // [K]
     const double *thr_k;     // [K] ascending
     int K;
     float *den1;             // [K][n] bin order
+    const int *order;        // nullable: waves heaviest first
 };
 
-__global__ __launch_bounds__(256) void k_loo_multi(LooMultiArgs A)
+// Per-wave batched leave-one-out densities: each wave owns 64 queries in bin
+// order and walks the candidate bins of its own bounding box; a source's
+// position and its K coefficients are wave-uniform and come in by scalar
+// loads.  Candidates are visited from the largest bandwidth down and the
+// walk stops at the first one with no lane inside its (upper) cut-off.
+// FP32 screen; a 32-source chunk with any lane inside a 2^-20 band is redone
+// with the exact FP64 test.
+__global__ __launch_bounds__(256) void k_loo_multi_w(LooMultiArgs A)
 {
-    __shared__ int s_bins[256];
-    __shared__ int s_pref[257];
-    __shared__ int s_w[4];
-    __shared__ float2 s_xy[256];
-    __shared__ int s_idx[256];
-    __shared__ float s_coef[LM_K][256];
-    __shared__ double s_tk[LM_K];
-    __shared__ float s_lo[LM_K], s_hi[LM_K], s_af[LM_K];
-    __shared__ float s_red[4][4];
-
     const PointIndex &I = A.idx;
-    const int tid = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int K = A.K;
     const int64_t n = I.n;
-    const int64_t q = (int64_t)blockIdx.x * 256 + tid;
+    const int64_t nw = (n + 63) / 64;
+    int64_t wv = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= nw) return;
+    if (A.order) wv = A.order[wv];
+    const int64_t q0 = wv * 64;
+    const int64_t q = q0 + lane;
     const bool qv = q < n;
     float qx = 0.0f, qy = 0.0f;
     if (qv) {
@@ -1091,99 +1153,73 @@ __global__ __launch_bounds__(256) void k_loo_multi(LooMultiArgs A)
         qx = p.x;
         qy = p.y;
     }
-    if (tid < LM_K) {
-        const double t = tid < K ? A.thr_k[tid] : -1.0;
-        s_tk[tid] = t;
-        s_lo[tid] = (float)(t * (1.0 - PD_BAND));
-        s_hi[tid] = tid < K ? fmaxf((float)(t * (1.0 + PD_BAND)), FLT_MIN) : -1.0f;
-        s_af[tid] = tid < K ? (float)A.a_k[tid] : 0.0f;
+    float lo[LM_K], hi[LM_K], af[LM_K];
+#pragma unroll
+    for (int k = 0; k < LM_K; ++k) {
+        const double t = k < K ? A.thr_k[k] : -1.0;
+        lo[k] = uni((float)(t * (1.0 - PD_BAND)));
+        hi[k] = uni(k < K ? fmaxf((float)(t * (1.0 + PD_BAND)), FLT_MIN) : -1.0f);
+        af[k] = uni(k < K ? (float)A.a_k[k] : 0.0f);
     }
-    float mnx, mxx, mny, mxy;
-    block_bbox(qv, qx, qy, s_red, mnx, mxx, mny, mxy);
-    const double tmax = s_tk[K - 1];
-
+    const double tmax = A.thr_k[K - 1];
+    const float mnx = uni(wave_min(qv ? qx : FLT_MAX)), mxx = uni(wave_max(qv ? qx : -FLT_MAX));
+    const float mny = uni(wave_min(qv ? qy : FLT_MAX)), mxy = uni(wave_max(qv ? qy : -FLT_MAX));
     int bxa, bxb, bya, byb;
     bin_range(I.bg, mnx, mxx, mny, mxy, sqrt(tmax) * 1.000001 + 1e-6, bxa, bxb, bya, byb);
-    const int nbx_r = bxb - bxa + 1;
-    const int nb_total = nbx_r * (byb - bya + 1);
+    bxa = __builtin_amdgcn_readfirstlane(bxa);
+    bxb = __builtin_amdgcn_readfirstlane(bxb);
+    bya = __builtin_amdgcn_readfirstlane(bya);
+    byb = __builtin_amdgcn_readfirstlane(byb);
 
     float acc[LM_K], sum[LM_K], comp[LM_K];
 #pragma unroll
     for (int k = 0; k < LM_K; ++k) { acc[k] = 0.0f; sum[k] = 0.0f; comp[k] = 0.0f; }
-
-    for (int base = 0; base < nb_total; base += 256) {
-        const int kk = base + tid;
-        int cntb = 0;
-        int b = 0;
-        if (kk < nb_total) {
-            b = (bya + kk / nbx_r) * I.bg.nbx + (bxa + kk % nbx_r);
-            const int c = I.bin_start[b + 1] - I.bin_start[b];
-            if (c > 0 && box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < tmax) cntb = c;
-        }
-        int nb;
-        const int slot = block_compact256(cntb > 0, s_w, &nb);
-        if (cntb > 0) s_bins[slot] = b;
-        __syncthreads();
-        int T;
-        const int mycnt = tid < nb ? I.bin_start[s_bins[tid] + 1] - I.bin_start[s_bins[tid]] : 0;
-        const int pre = block_exscan256(mycnt, s_w, &T);
-        if (tid < nb) s_pref[tid] = pre;
-        if (tid == 0) s_pref[nb] = T;
-        __syncthreads();
-        for (int g0 = 0; g0 < T; g0 += 256) {
-            const int gidx = g0 + tid;
-            const int cnt = (T - g0) < 256 ? (T - g0) : 256;
-            if (gidx < T) {
-                int lo = 0, hi = nb - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_pref[mid] <= gidx) lo = mid;
-                    else hi = mid - 1;
-                }
-                const int si = I.bin_start[s_bins[lo]] + (gidx - s_pref[lo]);
-                const PtRec p = I.pts[si];
-                s_xy[tid] = make_float2(p.x, p.y);
-                s_idx[tid] = si;
-#pragma unroll
-                for (int k = 0; k < LM_K; ++k) s_coef[k][tid] = k < K ? A.coefk[(int64_t)k * n + si] : 0.0f;
-            }
-            __syncthreads();
-            for (int s0 = 0; s0 < cnt; s0 += 32) {
-                const int s1 = (s0 + 32 < cnt) ? s0 + 32 : cnt;
+    for (int by = bya; by <= byb; ++by) {
+        for (int bx = bxa; bx <= bxb; ++bx) {
+            const int b = by * I.bg.nbx + bx;
+            const int st = I.bin_start[b], en = I.bin_start[b + 1];
+            if (st == en) continue;
+            if (!__builtin_amdgcn_readfirstlane((int)(box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < tmax))) continue;
+            for (int c0 = st; c0 < en; c0 += 32) {
+                const int c1 = c0 + 32 < en ? c0 + 32 : en;
                 bool band = false;
-                for (int s = s0; s < s1; ++s) {
-                    const float2 sp = s_xy[s];
-                    const float dx = sp.x - qx, dy = sp.y - qy;
+                for (int si = c0; si < c1; ++si) {
+                    const PtRec p = I.pts[si];
+                    const float4 *cp = (const float4 *)(A.coefk + (int64_t)si * LM_K);
+                    const float4 cf0 = cp[0], cf1 = cp[1], cf2 = cp[2], cf3 = cp[3];
+                    const float cf[LM_K] = {cf0.x, cf0.y, cf0.z, cf0.w, cf1.x, cf1.y, cf1.z, cf1.w,
+                                            cf2.x, cf2.y, cf2.z, cf2.w, cf3.x, cf3.y, cf3.z, cf3.w};
+                    const float dx = p.x - qx, dy = p.y - qy;
                     const float d2f = __builtin_fmaf(dy, dy, dx * dx);
-                    const bool self = (int64_t)s_idx[s] == q;
+                    const bool self = (int64_t)si == q;
 #pragma unroll
                     for (int k = LM_K - 1; k >= 0; --k) {
                         if (k < K) {
-                            const bool maybe = d2f < s_hi[k];
+                            const bool maybe = d2f < hi[k];
                             if (!__any(maybe)) break;
-                            const bool in = d2f < s_lo[k];
+                            const bool in = d2f < lo[k];
                             band |= maybe != in;
-                            const float e = fast_exp2(d2f * s_af[k]);
-                            acc[k] = __builtin_fmaf((in && !self) ? e : 0.0f, s_coef[k][s], acc[k]);
+                            const float e = fast_exp2(d2f * af[k]);
+                            acc[k] = __builtin_fmaf((in && !self) ? e : 0.0f, cf[k], acc[k]);
                         }
                     }
                 }
-                if (__any(band)) {
-                    // exact FP64 cut-off for this chunk (rare)
+                if (__any(band)) {   // rare: exact FP64 cut-off for the chunk
 #pragma unroll
                     for (int k = 0; k < LM_K; ++k) acc[k] = 0.0f;
-                    for (int s = s0; s < s1; ++s) {
-                        const float2 sp = s_xy[s];
-                        const float dx = sp.x - qx, dy = sp.y - qy;
+                    for (int si = c0; si < c1; ++si) {
+                        const PtRec p = I.pts[si];
+                        const float *cq = A.coefk + (int64_t)si * LM_K;
+                        const float dx = p.x - qx, dy = p.y - qy;
                         const float d2f = __builtin_fmaf(dy, dy, dx * dx);
                         const double d2 = sq2(dx, dy);
-                        const bool self = (int64_t)s_idx[s] == q;
+                        const bool self = (int64_t)si == q;
 #pragma unroll
                         for (int k = 0; k < LM_K; ++k) {
                             if (k < K) {
-                                const bool in = d2 < s_tk[k] && !self;
-                                const float e = fast_exp2(d2f * s_af[k]);
-                                acc[k] = __builtin_fmaf(in ? e : 0.0f, s_coef[k][s], acc[k]);
+                                const bool in = d2 < A.thr_k[k] && !self;
+                                const float e = fast_exp2(d2f * af[k]);
+                                acc[k] = __builtin_fmaf(in ? e : 0.0f, cq[k], acc[k]);
                             }
                         }
                     }
@@ -1194,13 +1230,13 @@ __global__ __launch_bounds__(256) void k_loo_multi(LooMultiArgs A)
                     acc[k] = 0.0f;
                 }
             }
-            __syncthreads();
         }
     }
 #pragma unroll
     for (int k = 0; k < LM_K; ++k)
         if (qv && k < K) A.den1[(int64_t)k * n + q] = sum[k];
 }
+
 
 // ==================================================== range candidates ====
 // Candidate sets of N1 with the QUERY's own threshold (tests only).  Pass 1
@@ -1832,13 +1868,14 @@ __global__ void k_sort_f32(int64_t n, const int *perm, const float *src, float *
 }
 
 // coefk[k*n + i] = w * ew_k / (2 pi h2_k), bin order
+// coef_out[i * stride]: one candidate's coefficient of every point (bin order)
 __global__ void k_coef_k(int64_t n, const int *perm, const float *w_orig, const float *ew_k_orig, double h2k,
-                         float *coef_out)
+                         float *coef_out, int stride)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int o = perm[i];
-    coef_out[i] = (float)((double)w_orig[o] * (double)ew_k_orig[o] / (h2k * (double)MKDE_TWO_PI_F));
+    coef_out[i * stride] = (float)((double)w_orig[o] * (double)ew_k_orig[o] / (h2k * (double)MKDE_TWO_PI_F));
 }
 
 __global__ void k_fill_f32(int64_t n, float v, float *dst)
@@ -1942,32 +1979,47 @@ hipError_t density_merge(hipStream_t s, const GridDesc &g, const Tiling &tl, con
     return hipGetLastError();
 }
 
-hipError_t point_density(hipStream_t s, const PointIndex &idx, int loo, float *den_sorted, float h_fixed)
+// Heaviest-first order of the 64-query waves (cost, hist: scratch; order: nw)
+static void wave_order(hipStream_t s, const PointIndex &idx, double lim_fixed, const double *lim_dev, unsigned *cost,
+                       unsigned *hist, int *order)
 {
-    PtsArgs A{idx, den_sorted, loo, 0.0, 0.0};
+    const int nw = (int)((idx.n + 63) / 64);
+    hipLaunchKernelGGL(k_wave_cost, dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, s, idx, lim_fixed, lim_dev, cost);
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, (int64_t)32, 0.0f, (float *)hist);
+    hipLaunchKernelGGL(k_cost_hist, dim3(nblk_(nw)), dim3(256), 0, s, nw, cost, hist);
+    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(64), 0, s, hist);
+    hipLaunchKernelGGL(k_cost_order, dim3(nblk_(nw)), dim3(256), 0, s, nw, cost, hist, order);
+}
+
+hipError_t point_density(hipStream_t s, const PointIndex &idx, int loo, float *den_sorted, float h_fixed,
+                         unsigned *wcost, unsigned *whist, int *worder)
+{
+    PtsArgs A{idx, den_sorted, loo, 0.0, 0.0, worder};
     if (idx.n <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)(((idx.n + 63) / 64 + 3) / 4);
     if (h_fixed > 0.0f) {
         const double h2 = (double)(float)(h_fixed * h_fixed);
         A.thr_fixed = 9.0 * h2;
         A.a_fixed = -MKDE_LOG2E / (2.0 * h2);
-        if (loo) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density<true, true>), dim3(nblk(idx.n, 256)), dim3(256), 0,
-                                    s, A);
-        else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density<true, false>), dim3(nblk(idx.n, 256)), dim3(256), 0,
-                                s, A);
+        if (worder) wave_order(s, idx, A.thr_fixed, nullptr, wcost, whist, worder);
+        if (loo) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density_w<true, true>), dim3(grid), dim3(256), 0, s, A);
+        else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density_w<true, false>), dim3(grid), dim3(256), 0, s, A);
     } else {
-        if (loo) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density<false, true>), dim3(nblk(idx.n, 256)), dim3(256),
-                                    0, s, A);
-        else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density<false, false>), dim3(nblk(idx.n, 256)), dim3(256),
-                                0, s, A);
+        if (worder) wave_order(s, idx, 0.0, nullptr, wcost, whist, worder);
+        if (loo) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density_w<false, true>), dim3(grid), dim3(256), 0, s, A);
+        else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density_w<false, false>), dim3(grid), dim3(256), 0, s, A);
     }
     return hipGetLastError();
 }
 
 hipError_t loo_multi(hipStream_t s, const PointIndex &idx, const float *coefk, const double *a_k,
-                     const double *thr_k, int K, float *den1)
+                     const double *thr_k, int K, float *den1, unsigned *wcost, unsigned *whist, int *worder)
 {
-    LooMultiArgs A{idx, coefk, a_k, thr_k, K, den1};
-    if (idx.n > 0) hipLaunchKernelGGL(k_loo_multi, dim3(nblk(idx.n, 256)), dim3(256), 0, s, A);
+    LooMultiArgs A{idx, coefk, a_k, thr_k, K, den1, worder};
+    if (idx.n <= 0) return hipSuccess;
+    if (worder) wave_order(s, idx, 0.0, thr_k + (K - 1), wcost, whist, worder);
+    const unsigned grid = (unsigned)(((idx.n + 63) / 64 + 3) / 4);
+    hipLaunchKernelGGL(k_loo_multi_w, dim3(grid), dim3(256), 0, s, A);
     return hipGetLastError();
 }
 
@@ -2089,9 +2141,10 @@ hipError_t sort_f32(hipStream_t s, int64_t n, const int *perm, const float *src,
 }
 
 hipError_t coef_k(hipStream_t s, int64_t n, const int *perm, const float *w, const float *ew_k, double h2k,
-                  float *coef_out)
+                  float *coef_out, int stride)
 {
-    if (n > 0) hipLaunchKernelGGL(k_coef_k, dim3(nblk(n, 256)), dim3(256), 0, s, n, perm, w, ew_k, h2k, coef_out);
+    if (n > 0)
+        hipLaunchKernelGGL(k_coef_k, dim3(nblk(n, 256)), dim3(256), 0, s, n, perm, w, ew_k, h2k, coef_out, stride);
     return hipGetLastError();
 }
 
