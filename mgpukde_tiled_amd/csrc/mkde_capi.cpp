@@ -36,9 +36,12 @@ hipError_t density_tile(hipStream_t, const GridDesc &, const float *, float *, c
                         const Tiling &);
 hipError_t density_merge(hipStream_t, const GridDesc &, const Tiling &, const float *, int, const int4 *,
                          const double *, float *);
-hipError_t point_density(hipStream_t, const PointIndex &, int, float *, float, unsigned *, unsigned *, int *);
+hipError_t point_plan(hipStream_t, const PointIndex &, int, float, const double *, int, const DensityWork &);
+hipError_t point_density(hipStream_t, const PointIndex &, int, float *, float, const int *, const int4 *,
+                         const unsigned *, double *, int);
 hipError_t loo_multi(hipStream_t, const PointIndex &, const float *, const double *, const double *, int, float *,
-                     unsigned *, unsigned *, int *);
+                     const int *, const int4 *, const unsigned *, double *, int);
+hipError_t point_merge(hipStream_t, int64_t, int, int, int, const int4 *, const double *, float *);
 hipError_t range_count(hipStream_t, const PointIndex &, int64_t *);
 hipError_t range_fill(hipStream_t, const PointIndex &, const int64_t *, int32_t *);
 hipError_t valid_cells(hipStream_t, int64_t, const float *, float, uint8_t *);
@@ -200,9 +203,12 @@ struct mkde_ctx {
     DevBuf<float> coefk, ewk, den1k;
     DevBuf<double> ak, tk;
     DevBuf<float> hsave, ewsave;   // bandwidths / EC weights saved across a likelihood scan
-    // heaviest-first order of the 64-query waves of the point kernels
-    DevBuf<unsigned> pw_cost, pw_hist;
+    // work list of the point kernels (64-query waves, heavy ones split)
+    DevBuf<unsigned> pw_cost, pw_parts, pw_item0, pw_slot0, pw_sidx, pw_sidx2, pw_chunks, pw_counts, pw_hist;
+    DevBuf<unsigned long long> pw_total;
+    DevBuf<int4> pw_items, pw_splits;
     DevBuf<int> pw_order;
+    DevBuf<double> pw_partials;
     // density kernel counters (mkde_density_stats)
     bool dt_stats_on = false;
     DevBuf<unsigned long long> dt_stats;
@@ -813,12 +819,43 @@ extern "C" int mkde_range_candidates(mkde_ctx *c, int64_t *offsets, int32_t *idx
 
 // ------------------------------------------------- densities at points --
 
-static int ensure_wave_order(mkde_ctx *c)
+// Work list of the point kernels: plan on the device, read the three counts,
+// then items, heaviest-first order and partial slots (m values per query).
+struct PointWork {
+    const int *order = nullptr;
+    const int4 *items = nullptr;
+    int n_items = 0, n_split = 0;
+};
+
+static int plan_points(mkde_ctx *c, const PointIndex &I, int kind, float h_fixed, const double *thr_k, int K, int m,
+                       PointWork &P)
 {
     const int64_t nw = (c->n + 63) / 64;
+    if (nw == 0) return MKDE_OK;
     CHK(c->pw_cost.ensure(nw));
+    CHK(c->pw_parts.ensure(nw));
+    CHK(c->pw_item0.ensure(nw));
+    CHK(c->pw_slot0.ensure(nw));
+    CHK(c->pw_sidx.ensure(nw));
+    CHK(c->pw_sidx2.ensure(nw));
+    CHK(c->pw_chunks.ensure((nw + 1023) / 1024));
+    CHK(c->pw_counts.ensure(3));
     CHK(c->pw_hist.ensure(32));
-    CHK(c->pw_order.ensure(nw));
+    CHK(c->pw_total.ensure(1));
+    DensityWork W{c->pw_cost.p,  c->pw_parts.p,  c->pw_item0.p, c->pw_slot0.p, c->pw_sidx.p,
+                  c->pw_sidx2.p, c->pw_chunks.p, c->pw_total.p, c->pw_counts.p, c->pw_hist.p};
+    HIPCHK(mkde_dev::point_plan(c->stream, I, kind, h_fixed, thr_k, K, W));
+    unsigned cnt[3];
+    CHK(copy_d2h(c, cnt, c->pw_counts.p, sizeof cnt));
+    P.n_items = (int)cnt[0];
+    P.n_split = (int)cnt[2];
+    CHK(c->pw_items.ensure(P.n_items));
+    CHK(c->pw_order.ensure(P.n_items));
+    CHK(c->pw_splits.ensure(P.n_split));
+    CHK(c->pw_partials.ensure((size_t)cnt[1] * 64 * m));
+    HIPCHK(mkde_dev::density_items(c->stream, (int)nw, W, P.n_items, c->pw_items.p, c->pw_splits.p, c->pw_order.p));
+    P.order = c->pw_order.p;
+    P.items = c->pw_items.p;
     return MKDE_OK;
 }
 
@@ -827,9 +864,11 @@ static int density_points_sorted(mkde_ctx *c, int loo, float *den_sorted)
     CHK(prep(c));
     PointIndex I = index_of(c);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    CHK(ensure_wave_order(c));
-    HIPCHK(mkde_dev::point_density(c->stream, I, loo, den_sorted, c->h_fixed, c->pw_cost.p, c->pw_hist.p,
-                                   c->pw_order.p));
+    PointWork P;
+    CHK(plan_points(c, I, 0, c->h_fixed, nullptr, 0, 1, P));
+    HIPCHK(mkde_dev::point_density(c->stream, I, loo, den_sorted, c->h_fixed, P.order, P.items, c->pw_cost.p,
+                                   c->pw_partials.p, P.n_items));
+    HIPCHK(mkde_dev::point_merge(c->stream, c->n, 1, 1, P.n_split, c->pw_splits.p, c->pw_partials.p, den_sorted));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     return MKDE_OK;
 }
@@ -905,9 +944,12 @@ extern "C" int mkde_loo_loglik(mkde_ctx *c, int K, const float *h_cand, int ec_o
         CHK(copy_h2d(c, c->ak.p, a.data(), kg * sizeof(double)));
         CHK(copy_h2d(c, c->tk.p, t.data(), kg * sizeof(double)));
         HIPCHK(hipEventRecord(c->ev0, c->stream));
-        CHK(ensure_wave_order(c));
-        HIPCHK(mkde_dev::loo_multi(c->stream, I, c->coefk.p, c->ak.p, c->tk.p, kg, c->den1k.p, c->pw_cost.p,
-                                   c->pw_hist.p, c->pw_order.p));
+        PointWork P;
+        CHK(plan_points(c, I, 1, 0.0f, c->tk.p, kg, MKDE_LM_K, P));
+        HIPCHK(mkde_dev::loo_multi(c->stream, I, c->coefk.p, c->ak.p, c->tk.p, kg, c->den1k.p, P.order, P.items,
+                                   c->pw_cost.p, c->pw_partials.p, P.n_items));
+        HIPCHK(mkde_dev::point_merge(c->stream, n, MKDE_LM_K, kg, P.n_split, c->pw_splits.p, c->pw_partials.p,
+                                     c->den1k.p));
         HIPCHK(hipEventRecord(c->ev1, c->stream));
         CHK(sum_logs(c, c->den1k.p, kg, 1, c->dsum.p));
         std::vector<double> part(kg);

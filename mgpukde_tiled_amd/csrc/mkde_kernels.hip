@@ -780,18 +780,20 @@ __global__ void k_cost_total(int n, const unsigned *cost, unsigned long long *to
     if (threadIdx.x == 0) atomicAdd(total, s_w[0] + s_w[1] + s_w[2] + s_w[3]);
 }
 
-__device__ __forceinline__ unsigned split_threshold(unsigned long long total)
+__device__ __forceinline__ unsigned split_threshold(unsigned long long total, unsigned tmin, unsigned long long target)
 {
-    const unsigned long long t = (total + DT_TARGET - 1) / DT_TARGET;
-    return t > DT_TMIN ? (unsigned)(t < 0xffffffffull ? t : 0xffffffffull) : DT_TMIN;
+    const unsigned long long t = (total + target - 1) / target;
+    return t > tmin ? (unsigned)(t < 0xffffffffull ? t : 0xffffffffull) : tmin;
 }
 
-__global__ void k_parts(int n, const unsigned *cost, const unsigned long long *total, unsigned *parts,
-                        unsigned *sparts, unsigned *sflag)
+// Split parts of each unit: T = max(tmin, ceil(total / target)), parts =
+// cost > T ? ceil(cost / T) : 1.
+__global__ void k_parts(int n, const unsigned *cost, const unsigned long long *total, unsigned tmin,
+                        unsigned long long target, unsigned *parts, unsigned *sparts, unsigned *sflag)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n) return;
-    const unsigned T = split_threshold(*total);
+    const unsigned T = split_threshold(*total, tmin, target);
     const unsigned c = cost[b];
     const unsigned k = c > T ? (unsigned)(((unsigned long long)c + T - 1) / T) : 1u;
     parts[b] = k;
@@ -909,7 +911,14 @@ struct PtsArgs {
     float *den;   // output, bin order
     int loo;
     double thr_fixed, a_fixed;   // FIXED kernels only
-    const int *order;            // nullable: waves heaviest first
+    // work items (nullable order: one item per wave): {wave, part j, parts K,
+    // partial slot}; part j of a wave takes candidate ordinals
+    // [C j / K, C (j+1) / K) of its C candidate sources (cost[wave])
+    const int *order;
+    const int4 *items;
+    const unsigned *cost;
+    double *partials;            // 64 per slot
+    int n_items;
 };
 
 #define PD_BAND (1.0 / 1048576.0)
@@ -971,28 +980,21 @@ __global__ __launch_bounds__(256) void k_wave_cost(PointIndex I, double lim_fixe
     if (lane == 0) cost[wv] = acc;
 }
 
-__global__ void k_cost_hist(int n, const unsigned *cost, unsigned *hist)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&hist[cost_bucket(cost[i])], 1u);
-}
-
-__global__ void k_cost_order(int n, const unsigned *cost, unsigned *offs, int *order)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) order[atomicAdd(&offs[cost_bucket(cost[i])], 1u)] = i;
-}
-
 template <bool FIXED, bool LOO>
 __global__ __launch_bounds__(256) void k_point_density_w(PtsArgs A)
 {
     const PointIndex &I = A.idx;
     const int lane = threadIdx.x & 63;
-    const int64_t nw = (I.n + 63) / 64;
-    int64_t wv = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wv >= nw) return;   // whole wave
-    if (A.order) wv = A.order[wv];
-    const int64_t q0 = wv * 64;
+    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= (A.order ? (int64_t)A.n_items : (I.n + 63) / 64)) return;   // whole wave
+    const int4 item = A.order ? A.items[A.order[g]] : make_int4((int)g, 0, 1, -1);
+    int win0 = 0, win1 = INT_MAX;   // candidate counts are below 2^31
+    if (item.z > 1) {
+        const int64_t C = A.cost[item.x];
+        if (item.y > 0) win0 = (int)(C * item.y / item.z);
+        if (item.y < item.z - 1) win1 = (int)(C * (item.y + 1) / item.z);
+    }
+    const int64_t q0 = (int64_t)item.x * 64;
     const int64_t q = q0 + lane;
     const bool qv = q < I.n;
     float qx = 0.0f, qy = 0.0f;
@@ -1015,13 +1017,20 @@ __global__ __launch_bounds__(256) void k_point_density_w(PtsArgs A)
     const float f_thr_hi = FIXED ? fmaxf((float)(A.thr_fixed * (1.0 + PD_BAND)), FLT_MIN) : 0.0f;
     const float f_a = FIXED ? (float)A.a_fixed : 0.0f;
     float acc = 0.0f, sum = 0.0f, comp = 0.0f;
+    int ord = 0;   // candidate ordinal at the start of the next candidate bin
     for (int by = bya; by <= byb; ++by) {
         for (int bx = bxa; bx <= bxb; ++bx) {
             const int b = by * I.bg.nbx + bx;
-            const int st = I.bin_start[b], en = I.bin_start[b + 1];
-            if (st == en) continue;
+            const int bst = I.bin_start[b], ben = I.bin_start[b + 1];
+            if (bst == ben) continue;
             const double lim = FIXED ? A.thr_fixed : I.bin_thr[b];
             if (!__builtin_amdgcn_readfirstlane((int)(box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < lim))) continue;
+            // this part's share of the candidate sequence
+            const int o0 = ord, o1 = ord + (ben - bst);
+            ord = o1;
+            if (o1 <= win0 || o0 >= win1) continue;
+            const int st = bst + (win0 > o0 ? win0 - o0 : 0);
+            const int en = ben - (o1 > win1 ? o1 - win1 : 0);
             for (int c0 = st; c0 < en; c0 += 32) {
                 const int c1 = c0 + 32 < en ? c0 + 32 : en;
                 // four records are loaded before their arithmetic, so the
@@ -1082,7 +1091,8 @@ __global__ __launch_bounds__(256) void k_point_density_w(PtsArgs A)
             }
         }
     }
-    if (qv) A.den[q] = sum;
+    if (item.z > 1) A.partials[(int64_t)item.w * 64 + lane] = (double)sum - (double)comp;
+    else if (qv) A.den[q] = sum;
 }
 
 // Exclusive scan of one int per thread across a 256-thread block.
@@ -1124,7 +1134,11 @@ struct LooMultiArgs {
     const double *thr_k;     // [K] ascending
     int K;
     float *den1;             // [K][n] bin order
-    const int *order;        // nullable: waves heaviest first
+    const int *order;        // work items as for the point kernel
+    const int4 *items;
+    const unsigned *cost;
+    double *partials;        // 64 * MKDE_LM_K per slot
+    int n_items;
 };
 
 // Per-wave batched leave-one-out densities: each wave owns 64 queries in bin
@@ -1140,11 +1154,16 @@ __global__ __launch_bounds__(256) void k_loo_multi_w(LooMultiArgs A)
     const int lane = threadIdx.x & 63;
     const int K = A.K;
     const int64_t n = I.n;
-    const int64_t nw = (n + 63) / 64;
-    int64_t wv = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wv >= nw) return;
-    if (A.order) wv = A.order[wv];
-    const int64_t q0 = wv * 64;
+    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= (A.order ? (int64_t)A.n_items : (n + 63) / 64)) return;
+    const int4 item = A.order ? A.items[A.order[g]] : make_int4((int)g, 0, 1, -1);
+    int win0 = 0, win1 = INT_MAX;   // candidate counts are below 2^31
+    if (item.z > 1) {
+        const int64_t C = A.cost[item.x];
+        if (item.y > 0) win0 = (int)(C * item.y / item.z);
+        if (item.y < item.z - 1) win1 = (int)(C * (item.y + 1) / item.z);
+    }
+    const int64_t q0 = (int64_t)item.x * 64;
     const int64_t q = q0 + lane;
     const bool qv = q < n;
     float qx = 0.0f, qy = 0.0f;
@@ -1153,12 +1172,14 @@ __global__ __launch_bounds__(256) void k_loo_multi_w(LooMultiArgs A)
         qx = p.x;
         qy = p.y;
     }
-    float lo[LM_K], hi[LM_K], af[LM_K];
+    // one FP32 threshold per candidate; the 2^-20 band is applied to the
+    // source's d2f instead (d2f * (1 + 2^-20) < t: inside for sure;
+    // d2f * (1 - 2^-20) >= t: outside for sure), which keeps the candidate
+    // constants within the scalar registers
+    float tf[LM_K], af[LM_K];
 #pragma unroll
     for (int k = 0; k < LM_K; ++k) {
-        const double t = k < K ? A.thr_k[k] : -1.0;
-        lo[k] = uni((float)(t * (1.0 - PD_BAND)));
-        hi[k] = uni(k < K ? fmaxf((float)(t * (1.0 + PD_BAND)), FLT_MIN) : -1.0f);
+        tf[k] = uni(k < K ? (float)A.thr_k[k] : -1.0f);
         af[k] = uni(k < K ? (float)A.a_k[k] : 0.0f);
     }
     const double tmax = A.thr_k[K - 1];
@@ -1174,12 +1195,18 @@ __global__ __launch_bounds__(256) void k_loo_multi_w(LooMultiArgs A)
     float acc[LM_K], sum[LM_K], comp[LM_K];
 #pragma unroll
     for (int k = 0; k < LM_K; ++k) { acc[k] = 0.0f; sum[k] = 0.0f; comp[k] = 0.0f; }
+    int ord = 0;
     for (int by = bya; by <= byb; ++by) {
         for (int bx = bxa; bx <= bxb; ++bx) {
             const int b = by * I.bg.nbx + bx;
-            const int st = I.bin_start[b], en = I.bin_start[b + 1];
-            if (st == en) continue;
+            const int bst = I.bin_start[b], ben = I.bin_start[b + 1];
+            if (bst == ben) continue;
             if (!__builtin_amdgcn_readfirstlane((int)(box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < tmax))) continue;
+            const int o0 = ord, o1 = ord + (ben - bst);
+            ord = o1;
+            if (o1 <= win0 || o0 >= win1) continue;
+            const int st = bst + (win0 > o0 ? win0 - o0 : 0);
+            const int en = ben - (o1 > win1 ? o1 - win1 : 0);
             for (int c0 = st; c0 < en; c0 += 32) {
                 const int c1 = c0 + 32 < en ? c0 + 32 : en;
                 bool band = false;
@@ -1191,13 +1218,14 @@ __global__ __launch_bounds__(256) void k_loo_multi_w(LooMultiArgs A)
                                             cf2.x, cf2.y, cf2.z, cf2.w, cf3.x, cf3.y, cf3.z, cf3.w};
                     const float dx = p.x - qx, dy = p.y - qy;
                     const float d2f = __builtin_fmaf(dy, dy, dx * dx);
+                    const float d2_in = d2f * (float)(1.0 + PD_BAND), d2_maybe = d2f * (float)(1.0 - PD_BAND);
                     const bool self = (int64_t)si == q;
 #pragma unroll
                     for (int k = LM_K - 1; k >= 0; --k) {
                         if (k < K) {
-                            const bool maybe = d2f < hi[k];
+                            const bool maybe = d2_maybe < tf[k];
                             if (!__any(maybe)) break;
-                            const bool in = d2f < lo[k];
+                            const bool in = d2_in < tf[k];
                             band |= maybe != in;
                             const float e = fast_exp2(d2f * af[k]);
                             acc[k] = __builtin_fmaf((in && !self) ? e : 0.0f, cf[k], acc[k]);
@@ -1232,9 +1260,30 @@ __global__ __launch_bounds__(256) void k_loo_multi_w(LooMultiArgs A)
             }
         }
     }
+    if (item.z > 1) {
+        double *pp = A.partials + (int64_t)item.w * 64 * LM_K + lane;
 #pragma unroll
-    for (int k = 0; k < LM_K; ++k)
-        if (qv && k < K) A.den1[(int64_t)k * n + q] = sum[k];
+        for (int k = 0; k < LM_K; ++k) pp[k * 64] = (double)sum[k] - (double)comp[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < LM_K; ++k)
+            if (qv && k < K) A.den1[(int64_t)k * n + q] = sum[k];
+    }
+}
+
+// Parts of split waves: FP64 partials added in part order, rounded once.
+// splits[k] = {wave, slot0, parts}; m = values per query (1, or MKDE_LM_K).
+__global__ void k_point_merge(int64_t n, int m, int K, const int4 *splits, const double *partials, float *den)
+{
+    const int4 sp = splits[blockIdx.x];
+    const int lane = threadIdx.x & 63, kk = threadIdx.x >> 6;   // 64 x 4 threads
+    const int64_t q = (int64_t)sp.x * 64 + lane;
+    if (q >= n) return;
+    for (int k = kk; k < K; k += 4) {
+        double t = 0.0;
+        for (int j = 0; j < sp.z; ++j) t += partials[((int64_t)(sp.y + j) * m + k) * 64 + lane];
+        den[(int64_t)k * n + q] = (float)t;
+    }
 }
 
 
@@ -1934,8 +1983,8 @@ hipError_t density_plan(hipStream_t s, const GridDesc &g, const float *mask, con
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, (int64_t)2, 0.0f, (float *)W.total);
     const unsigned nred = (unsigned)std::min<int64_t>(1024, ((int64_t)nblk + 255) / 256);
     hipLaunchKernelGGL(k_cost_total, dim3(nred), dim3(256), 0, s, nblk, W.cost, W.total);
-    hipLaunchKernelGGL(k_parts, dim3(nblk_(nblk)), dim3(256), 0, s, nblk, W.cost, W.total, W.parts, W.item0,
-                       W.sidx);
+    hipLaunchKernelGGL(k_parts, dim3(nblk_(nblk)), dim3(256), 0, s, nblk, W.cost, W.total, DT_TMIN, DT_TARGET, W.parts,
+                       W.item0, W.sidx);
     // in place is not allowed: parts -> item0 scan reads parts; sparts / sflag
     // were written to item0 / sidx and are scanned into slot0 / sidx2 below
     scan_u32(s, nblk, W.item0, W.slot0, W.chunks, W.counts + 1);
@@ -1979,33 +2028,49 @@ hipError_t density_merge(hipStream_t s, const GridDesc &g, const Tiling &tl, con
     return hipGetLastError();
 }
 
-// Heaviest-first order of the 64-query waves (cost, hist: scratch; order: nw)
-static void wave_order(hipStream_t s, const PointIndex &idx, double lim_fixed, const double *lim_dev, unsigned *cost,
-                       unsigned *hist, int *order)
+// Work list of the point kernels (64-query waves): costs, split parts (T =
+// max(tmin, total / target)) and scans; W.counts = {items, slots, splits}.
+// kind 0: density at points (h_fixed > 0: one bandwidth), 1: batched
+// leave-one-out with thresholds thr_k[0..K-1] ascending.
+hipError_t point_plan(hipStream_t s, const PointIndex &idx, int kind, float h_fixed, const double *thr_k, int K,
+                      const DensityWork &W)
 {
     const int nw = (int)((idx.n + 63) / 64);
-    hipLaunchKernelGGL(k_wave_cost, dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, s, idx, lim_fixed, lim_dev, cost);
-    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, (int64_t)32, 0.0f, (float *)hist);
-    hipLaunchKernelGGL(k_cost_hist, dim3(nblk_(nw)), dim3(256), 0, s, nw, cost, hist);
-    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(64), 0, s, hist);
-    hipLaunchKernelGGL(k_cost_order, dim3(nblk_(nw)), dim3(256), 0, s, nw, cost, hist, order);
+    double lim_fixed = 0.0;
+    const double *lim_dev = nullptr;
+    if (kind == 1) {
+        lim_dev = thr_k + (K - 1);
+    } else if (h_fixed > 0.0f) {
+        lim_fixed = 9.0 * (double)(float)(h_fixed * h_fixed);
+    }
+    hipLaunchKernelGGL(k_wave_cost, dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, s, idx, lim_fixed, lim_dev, W.cost);
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, (int64_t)2, 0.0f, (float *)W.total);
+    const unsigned nred = (unsigned)std::min<int64_t>(1024, ((int64_t)nw + 255) / 256);
+    hipLaunchKernelGGL(k_cost_total, dim3(nred), dim3(256), 0, s, nw, W.cost, W.total);
+    const unsigned tmin = 4096;
+    const unsigned long long target = kind == 1 ? 2048ull : 16384ull;
+    hipLaunchKernelGGL(k_parts, dim3(nblk_(nw)), dim3(256), 0, s, nw, W.cost, W.total, tmin, target, W.parts,
+                       W.item0, W.sidx);
+    scan_u32(s, nw, W.item0, W.slot0, W.chunks, W.counts + 1);
+    scan_u32(s, nw, W.sidx, W.sidx2, W.chunks, W.counts + 2);
+    scan_u32(s, nw, W.parts, W.item0, W.chunks, W.counts + 0);
+    return hipGetLastError();
 }
 
 hipError_t point_density(hipStream_t s, const PointIndex &idx, int loo, float *den_sorted, float h_fixed,
-                         unsigned *wcost, unsigned *whist, int *worder)
+                         const int *order, const int4 *items, const unsigned *cost, double *partials, int n_items)
 {
-    PtsArgs A{idx, den_sorted, loo, 0.0, 0.0, worder};
+    PtsArgs A{idx, den_sorted, loo, 0.0, 0.0, order, items, cost, partials, n_items};
     if (idx.n <= 0) return hipSuccess;
-    const unsigned grid = (unsigned)(((idx.n + 63) / 64 + 3) / 4);
+    const int64_t units = order ? n_items : (idx.n + 63) / 64;
+    const unsigned grid = (unsigned)((units + 3) / 4);
     if (h_fixed > 0.0f) {
         const double h2 = (double)(float)(h_fixed * h_fixed);
         A.thr_fixed = 9.0 * h2;
         A.a_fixed = -MKDE_LOG2E / (2.0 * h2);
-        if (worder) wave_order(s, idx, A.thr_fixed, nullptr, wcost, whist, worder);
         if (loo) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density_w<true, true>), dim3(grid), dim3(256), 0, s, A);
         else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density_w<true, false>), dim3(grid), dim3(256), 0, s, A);
     } else {
-        if (worder) wave_order(s, idx, 0.0, nullptr, wcost, whist, worder);
         if (loo) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density_w<false, true>), dim3(grid), dim3(256), 0, s, A);
         else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_point_density_w<false, false>), dim3(grid), dim3(256), 0, s, A);
     }
@@ -2013,13 +2078,21 @@ hipError_t point_density(hipStream_t s, const PointIndex &idx, int loo, float *d
 }
 
 hipError_t loo_multi(hipStream_t s, const PointIndex &idx, const float *coefk, const double *a_k,
-                     const double *thr_k, int K, float *den1, unsigned *wcost, unsigned *whist, int *worder)
+                     const double *thr_k, int K, float *den1, const int *order, const int4 *items,
+                     const unsigned *cost, double *partials, int n_items)
 {
-    LooMultiArgs A{idx, coefk, a_k, thr_k, K, den1, worder};
+    LooMultiArgs A{idx, coefk, a_k, thr_k, K, den1, order, items, cost, partials, n_items};
     if (idx.n <= 0) return hipSuccess;
-    if (worder) wave_order(s, idx, 0.0, thr_k + (K - 1), wcost, whist, worder);
-    const unsigned grid = (unsigned)(((idx.n + 63) / 64 + 3) / 4);
-    hipLaunchKernelGGL(k_loo_multi_w, dim3(grid), dim3(256), 0, s, A);
+    const int64_t units = order ? n_items : (idx.n + 63) / 64;
+    hipLaunchKernelGGL(k_loo_multi_w, dim3((unsigned)((units + 3) / 4)), dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t point_merge(hipStream_t s, int64_t n, int m, int K, int n_split, const int4 *splits, const double *partials,
+                       float *den)
+{
+    if (n_split > 0)
+        hipLaunchKernelGGL(k_point_merge, dim3((unsigned)n_split), dim3(256), 0, s, n, m, K, splits, partials, den);
     return hipGetLastError();
 }
 
