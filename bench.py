@@ -44,6 +44,19 @@ def workload(scale=1.0):
     return x, y, w, mask, side
 
 
+def density_traffic():
+    """HBM bytes per launch of the density kernel on this workload, from the
+    two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE) scaled by the same
+    counters on a 1 GiB copy; None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_density_tile_thomas_adaptive_1M_8192_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":")[1]))
+    return None
+
+
 def cpu_baseline(x, y, w, ew, hs, mask, side, target_s=15.0):
     """The oracle's restatement of the reference CPU loop (brute force over all
     points per cell, float accumulation) on a seeded sample of 64x64 tiles."""
@@ -113,9 +126,11 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--stats", action="store_true", help="report density-kernel classification counters")
     ap.add_argument("--jobs", type=int, default=120, help="serve: number of jobs")
+    ap.add_argument("--serve-jobs", type=int, default=40,
+                    help="jobs of the serving sample added to the benchmark line (0: none)")
     ap.add_argument("--slots", type=int, default=4, help="serve: worker slots")
     ap.add_argument("--qps", type=float, default=0.0, help="serve: arrival rate (0: 4x the sequential rate)")
-    ap.add_argument("--workload", default="adaptive", choices=["adaptive", "search", "tiles", "serve"],
+    ap.add_argument("--workload", default="adaptive", choices=["adaptive", "search", "tiles", "serve", "copy"],
                     help="adaptive: the benchmark line; search / tiles: measurements of the other workloads")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -131,6 +146,8 @@ def main():
         return bench_tiles(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "copy":
+        return bench_copy(args)
     log("workload: generating inputs")
     x, y, w, mask, side = workload(args.scale)
     n = len(x)
@@ -150,6 +167,7 @@ def main():
     ctx.synchronize()
     wall = []
     stats = []
+    dens_ms = []   # density kernel time of every timed step (HIP events on the context's stream)
     t_all0 = time.perf_counter()
     for i in range(args.steps):
         t0 = time.perf_counter()
@@ -157,11 +175,11 @@ def main():
         ctx.synchronize()
         wall.append((time.perf_counter() - t0) * 1e3)
         stats.append(st)
+        dens_ms.append(ctx.last_kernel_ms("density"))
         log("step %d: %.1f ms (device %.1f ms; density %.1f ms)" % (i, wall[-1], st["ms_total"], st["ms_density"]))
     ctx.synchronize()
     t_all = time.perf_counter() - t_all0
     ms_per_step = t_all * 1e3 / args.steps
-    dens_ms = [ctx.last_kernel_ms("density")]
 
     dstats = None
     if args.stats:
@@ -186,8 +204,7 @@ def main():
     e_total = e_den + e_ec + e_ec0 + e_pilot
     log("work: E_den=%d E_ec=%d E_ec0=%d E_pilot=%d" % (e_den, e_ec, e_ec0, e_pilot))
 
-    dens_kernel_ms = float(np.median([s["ms_density"] for s in stats]))
-    ker_ms = dens_ms[0] if dens_ms[0] > 0 else dens_kernel_ms
+    ker_ms = float(np.mean(dens_ms))
     achieved_tflops = FLOP_PER_EVAL * e_den / (ker_ms * 1e-3) / 1e12
     valid_cells = int(np.count_nonzero(mask != -9999.0))
 
@@ -226,7 +243,7 @@ def main():
             "peak": MI355X_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / MI355X_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": density_traffic(),
             "kernel": "k_density_tile",
             "work_per_launch": "%d evaluations x %d FLOP" % (e_den, FLOP_PER_EVAL),
         },
@@ -241,6 +258,12 @@ def main():
     else:
         out["cpu_baseline"] = None
     ctx.close()
+    if args.serve_jobs > 0:
+        # job latency of the served mixed stream (a bounded sample of it)
+        args.jobs = args.serve_jobs
+        sv = bench_serve(args, emit=False)
+        out["serving"] = {k: sv[k] for k in ("jobs", "slots", "qps", "sequential_jobs_per_s", "served_jobs_per_s",
+                                             "speedup", "served_latency_ms", "served_run_ms", "identical_results")}
     print(json.dumps(out), flush=True)
 
 
@@ -405,7 +428,7 @@ def serve_jobs(count, seed=20260005, max_points=5_000_000, max_side=8192):
     return jobs
 
 
-def bench_serve(args):
+def bench_serve(args, emit=True):
     """Serving stream: the same job list run one job at a time (one slot,
     back to back) and served (several slots, Poisson arrivals at a fixed QPS);
     jobs/s, p50/p99 latency and bit-equality of every result."""
@@ -485,8 +508,28 @@ def bench_serve(args):
                                      for k, h in (("fixed", 0), ("adaptive", 2), ("search", 3))},
            "slowest_sequential": [{"points": jobs[i]["n"], "side": jobs[i]["side"], "hoption": jobs[i]["hoption"],
                                    "run_ms": seq[i][0]["run_ms"]} for i in np.argsort(-seq_run)[:6]]}
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
     return out
+
+
+def bench_copy(args):
+    """Calibration of the HBM byte counters: three device-to-device copies of
+    1 GiB (FETCH_SIZE / WRITE_SIZE of this copy against its known 1 GiB read
+    and 1 GiB write give the counters' scale)."""
+    import torch
+
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 3
+    print(json.dumps({"workload": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "ms": dt * 1e3,
+                      "GBps": 8.0 * n / dt / 1e9}), flush=True)
 
 
 def ctx_range_total(ctx):

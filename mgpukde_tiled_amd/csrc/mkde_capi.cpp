@@ -179,7 +179,8 @@ struct mkde_ctx {
     DevBuf<uint8_t> bflags;
     DevBuf<float2> blist;
     DevBuf<unsigned long long> bcount;
-    int64_t nboundary = -1;
+    bool bflags_ok = false;    // boundary flags / list enqueued for the current mask
+    int64_t nboundary = -1;    // boundary cell count once read back
     // boundary hierarchy (nearest-boundary search)
     DevBuf<uint4> bt_bits;
     DevBuf<float4> bt_box;
@@ -578,6 +579,7 @@ extern "C" int mkde_mask_upload(mkde_ctx *c, const mkde_grid *g, const float *va
     }
     HIPCHK(mkde_dev::valid_cells(c->stream, nc, c->mask.p, g->nodata, c->valid.p));
     c->mask_set = true;
+    c->bflags_ok = false;
     c->nboundary = -1;
     c->bt_ok = false;
     c->dist_set = false;
@@ -592,15 +594,26 @@ static int need_mask(mkde_ctx *c)
     return MKDE_OK;
 }
 
+// Boundary flags and list are enqueued without a host wait; the count is
+// read back only when a caller needs it (boundary_count).
 static int ensure_boundary(mkde_ctx *c)
 {
     CHK(need_mask(c));
-    if (c->nboundary >= 0) return MKDE_OK;
+    if (c->bflags_ok) return MKDE_OK;
     const int64_t nc = c->mg.ncols * c->mg.nrows;
     // upper bound of the boundary list: every valid cell
     CHK(c->blist.ensure(nc));
     HIPCHK(hipMemsetAsync(c->bcount.p, 0, sizeof(unsigned long long), c->stream));
     HIPCHK(mkde_dev::boundary(c->stream, c->mg, c->valid.p, c->bflags.p, c->blist.p, c->bcount.p));
+    c->bflags_ok = true;
+    c->nboundary = -1;
+    return MKDE_OK;
+}
+
+static int boundary_count(mkde_ctx *c)
+{
+    CHK(ensure_boundary(c));
+    if (c->nboundary >= 0) return MKDE_OK;
     unsigned long long nb = 0;
     CHK(copy_d2h(c, &nb, c->bcount.p, sizeof nb));
     c->nboundary = (int64_t)nb;
@@ -611,7 +624,7 @@ extern "C" int mkde_mark_boundary(mkde_ctx *c, uint8_t *flags, int64_t *n_bounda
 {
     if (!c) return fail(MKDE_E_ARG, "null context");
     CHK(set_device(c));
-    CHK(ensure_boundary(c));
+    CHK(boundary_count(c));
     if (flags) CHK(copy_d2h(c, flags, c->bflags.p, (size_t)(c->mg.ncols * c->mg.nrows)));
     if (n_boundary) *n_boundary = c->nboundary;
     return MKDE_OK;
@@ -656,6 +669,7 @@ static int ensure_dist(mkde_ctx *c)
         CHK(prep(c));
         HIPCHK(mkde_dev::dist_tree(c->stream, c->n, c->xs.p, c->ys.p, c->perm.p, c->mg, c->bt, c->dist.p));
     } else {
+        CHK(boundary_count(c));
         HIPCHK(mkde_dev::dist_boundary(c->stream, c->n, c->x.p, c->y.p, c->blist.p, c->nboundary, c->dist.p));
     }
     c->dist_set = true;
@@ -1378,15 +1392,20 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
     HIPCHK(hipEventCreate(&t0));
     HIPCHK(hipEventCreate(&t1));
     HIPCHK(hipEventRecord(t0, c->stream));
-    S.h0 = rule_of_thumb_host(c->hx, c->hy, c->hw);
     if (ec_on) {
         PhaseTimer pt(c->stream, &S.ms_boundary);
         if (o->recompute_boundary) {
+            c->bflags_ok = false;
             c->nboundary = -1;
             c->bt_ok = false;
             c->dist_set = false;
         }
         CHK(ensure_dist(c));
+        // the rule of thumb (sequential float sums on the host, in the
+        // reference's order) runs while the device computes the distances
+        S.h0 = rule_of_thumb_host(c->hx, c->hy, c->hw);
+    } else {
+        S.h0 = rule_of_thumb_host(c->hx, c->hy, c->hw);
     }
     if (o->hoption != -1) CHK(mkde_set_bandwidth_fixed(c, S.h0));
     unsigned long long ne = 0;

@@ -208,23 +208,36 @@ __global__ void k_prep_points(int64_t n, const int *perm, const float *xs, const
 }
 
 // Per-bin bounding box of the points and largest threshold.
-__global__ void k_bin_stats(int nbins, const int *bin_start, const PtRec *pts, BinInfo *box, double *thr)
+// One wave per bin (bins in dense clusters hold thousands of points):
+// lanes stride over the bin's points, then wave min / max reductions.
+__global__ __launch_bounds__(256) void k_bin_stats(int nbins, const int *bin_start, const PtRec *pts, BinInfo *box,
+                                                   double *thr)
 {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (b >= nbins) return;
-    int s = bin_start[b], e = bin_start[b + 1];
+    const int s = bin_start[b], e = bin_start[b + 1];
     float x0 = FLT_MAX, y0 = FLT_MAX, x1 = -FLT_MAX, y1 = -FLT_MAX;
     double t = -1.0;
-    for (int i = s; i < e; ++i) {
-        PtRec p = pts[i];
+    for (int i = s + lane; i < e; i += 64) {
+        const PtRec p = pts[i];
         x0 = fminf(x0, p.x);
         x1 = fmaxf(x1, p.x);
         y0 = fminf(y0, p.y);
         y1 = fmaxf(y1, p.y);
         t = fmax(t, p.thr);
     }
-    box[b] = BinInfo{x0, y0, x1, y1};
-    thr[b] = t;
+    for (int d = 32; d >= 1; d >>= 1) {
+        x0 = fminf(x0, __shfl_xor(x0, d, WAVE));
+        x1 = fmaxf(x1, __shfl_xor(x1, d, WAVE));
+        y0 = fminf(y0, __shfl_xor(y0, d, WAVE));
+        y1 = fmaxf(y1, __shfl_xor(y1, d, WAVE));
+        t = fmax(t, __shfl_xor(t, d, WAVE));
+    }
+    if (lane == 0) {
+        box[b] = BinInfo{x0, y0, x1, y1};
+        thr[b] = t;
+    }
 }
 
 // ---- wave-uniform row values by DPP broadcast
@@ -812,22 +825,41 @@ __global__ void k_items_fill(int n, const unsigned *parts, const unsigned *item0
     if (k > 1) splits[sidx[b]] = make_int4(b, (int)slot0[b], k, 0);
 }
 
-__global__ void k_item_hist(int n, const int4 *items, const unsigned *cost, unsigned *hist)
+// Heaviest-first order of the items: bucket counts and slots are reserved
+// once per workgroup (LDS counters, one global atomic per bucket), so the
+// few large buckets do not serialise every item on one address.
+__global__ __launch_bounds__(256) void k_item_hist(int n, const int4 *items, const unsigned *cost, unsigned *hist)
 {
+    __shared__ unsigned s_h[32];
+    if (threadIdx.x < 32) s_h[threadIdx.x] = 0;
+    __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         const int4 it = items[i];
-        atomicAdd(&hist[cost_bucket(cost[it.x] / (unsigned)it.z)], 1u);
+        atomicAdd(&s_h[cost_bucket(cost[it.x] / (unsigned)it.z)], 1u);
     }
+    __syncthreads();
+    if (threadIdx.x < 32 && s_h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_h[threadIdx.x]);
 }
 
-__global__ void k_item_order(int n, const int4 *items, const unsigned *cost, unsigned *offs, int *order)
+__global__ __launch_bounds__(256) void k_item_order(int n, const int4 *items, const unsigned *cost, unsigned *offs,
+                                                    int *order)
 {
+    __shared__ unsigned s_h[32], s_base[32];
+    if (threadIdx.x < 32) s_h[threadIdx.x] = 0;
+    __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int bk = 0;
+    unsigned r = 0;
     if (i < n) {
         const int4 it = items[i];
-        order[atomicAdd(&offs[cost_bucket(cost[it.x] / (unsigned)it.z)], 1u)] = i;
+        bk = cost_bucket(cost[it.x] / (unsigned)it.z);
+        r = atomicAdd(&s_h[bk], 1u);
     }
+    __syncthreads();
+    if (threadIdx.x < 32 && s_h[threadIdx.x]) s_base[threadIdx.x] = atomicAdd(&offs[threadIdx.x], s_h[threadIdx.x]);
+    __syncthreads();
+    if (i < n) order[s_base[bk] + r] = i;
 }
 
 // ---- exclusive scan of unsigned values (1024 per workgroup, two levels)
@@ -1951,7 +1983,7 @@ hipError_t prep_points(hipStream_t s, int64_t n, const int *perm, const float *x
 
 hipError_t bin_stats(hipStream_t s, int nbins, const int *bin_start, const PtRec *pts, BinInfo *box, double *thr)
 {
-    hipLaunchKernelGGL(k_bin_stats, dim3(nblk(nbins, 256)), dim3(256), 0, s, nbins, bin_start, pts, box, thr);
+    hipLaunchKernelGGL(k_bin_stats, dim3(nblk(nbins, 4)), dim3(256), 0, s, nbins, bin_start, pts, box, thr);
     return hipGetLastError();
 }
 
