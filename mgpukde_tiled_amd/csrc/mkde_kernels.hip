@@ -384,8 +384,10 @@ __device__ __forceinline__ BlockGeom block_geom(const GridDesc &g, const Tiling 
     return b;
 }
 
+// 4 waves per SIMD: at most 128 VGPRs (no spills) and 37.8 KB of LDS per
+// workgroup, 4 workgroups per CU.
 template <bool STATS>
-__global__ __launch_bounds__(256) void k_density_tile(TileArgs A)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_density_tile(TileArgs A)
 {
     __shared__ int s_bins[256];
     __shared__ int s_wcnt[4];
