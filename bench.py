@@ -130,6 +130,8 @@ def main():
                     help="jobs of the serving sample added to the benchmark line (0: none)")
     ap.add_argument("--slots", type=int, default=4, help="serve: worker slots")
     ap.add_argument("--qps", type=float, default=0.0, help="serve: arrival rate (0: 4x the sequential rate)")
+    ap.add_argument("--plain", action="store_true",
+                    help="serve: also time the plain form (one job at a time, one candidate per pass)")
     ap.add_argument("--workload", default="adaptive", choices=["adaptive", "search", "tiles", "serve", "copy"],
                     help="adaptive: the benchmark line; search / tiles: measurements of the other workloads")
     args = ap.parse_args()
@@ -261,9 +263,11 @@ def main():
     if args.serve_jobs > 0:
         # job latency of the served mixed stream (a bounded sample of it)
         args.jobs = args.serve_jobs
+        args.plain = True
         sv = bench_serve(args, emit=False)
         out["serving"] = {k: sv[k] for k in ("jobs", "slots", "qps", "sequential_jobs_per_s", "served_jobs_per_s",
-                                             "speedup", "served_latency_ms", "served_run_ms", "identical_results")}
+                                             "speedup", "served_latency_ms", "served_run_ms", "identical_results",
+                                             "plain")}
     print(json.dumps(out), flush=True)
 
 
@@ -441,9 +445,10 @@ def bench_serve(args, emit=True):
     jobs = serve_jobs(args.jobs)
     log("serve: %d points, %.3g cells in total" % (sum(j["n"] for j in jobs), sum(j["side"] ** 2 for j in jobs)))
 
-    def submit(srv, j):
+    def submit(srv, j, plain=False):
         return srv.submit(j["x"], j["y"], j["g"], j["mask"], hoption=j["hoption"], edge_correction=True,
-                          optimise=False, alpha=-0.5, n_candidates=16, cand_lo=0.1, cand_hi=10.0)
+                          optimise=False, alpha=-0.5, n_candidates=16, cand_lo=0.1, cand_hi=10.0,
+                          one_candidate_per_pass=plain)
 
     def digest(a):
         return hashlib.blake2b(a.tobytes(), digest_size=16).hexdigest()
@@ -463,6 +468,18 @@ def bench_serve(args, emit=True):
         t_seq = time.perf_counter() - t0
     seq_rate = len(jobs) / t_seq
     log("serve: sequential %.2f jobs/s (%.1f s)" % (seq_rate, t_seq))
+    # the plain form: one job at a time and one candidate bandwidth per
+    # leave-one-out pass (the reference's per-candidate LogLikelihood calls)
+    plain = []
+    if args.plain:
+        with M.Server(slots=1) as srv:
+            t0 = time.perf_counter()
+            for j in jobs:
+                res, out = srv.wait(submit(srv, j, plain=True))
+                assert res["status"] == 0, res["message"]
+                plain.append((res, out.copy() if j["hoption"] == 3 else None, digest(out)))
+            t_plain = time.perf_counter() - t0
+        log("serve: plain %.2f jobs/s (%.1f s)" % (len(jobs) / t_plain, t_plain))
     qps = args.qps if args.qps > 0 else 4.0 * seq_rate
     arr = np.cumsum(np.random.default_rng(20260006).exponential(1.0 / qps, len(jobs)))
     results = [None] * len(jobs)
@@ -496,12 +513,21 @@ def bench_serve(args, emit=True):
     run = np.array([r[0]["run_ms"] for r in results])
     seq_run = np.array([r[0]["run_ms"] for r in seq])
     same = sum(1 for a, b in zip(seq, results) if a[1] == b[1] and a[0]["h_sel"] == b[0]["h_sel"])
+    plain_info = None
+    if plain:
+        # plain vs batched: the same selected bandwidths; rasters bit-identical
+        # except where the summation order of a candidate pass differs
+        same_h = sum(1 for a, b in zip(plain, seq) if a[0]["h_sel"] == b[0]["h_sel"])
+        same_r = sum(1 for a, b in zip(plain, seq) if a[2] == b[1])
+        plain_info = {"plain_jobs_per_s": len(jobs) / t_plain, "served_vs_plain": (len(jobs) / t_served) /
+                      (len(jobs) / t_plain), "plain_same_selected_h": same_h, "plain_identical_rasters": same_r}
     out = {"workload": "serve", "jobs": len(jobs), "slots": args.slots, "qps": qps,
            "sequential_jobs_per_s": seq_rate, "served_jobs_per_s": served_rate, "speedup": served_rate / seq_rate,
            "served_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
            "served_run_ms": {"p50": float(np.percentile(run, 50)), "p99": float(np.percentile(run, 99))},
            "sequential_run_ms": {"p50": float(np.percentile(seq_run, 50)), "p99": float(np.percentile(seq_run, 99))},
-           "identical_results": same, "peak_committed_bytes": info["peak_committed"], "budget_bytes": info["budget"],
+           "identical_results": same, "plain": plain_info,
+           "peak_committed_bytes": info["peak_committed"], "budget_bytes": info["budget"],
            "mix": {"fixed": sum(j["hoption"] == 0 for j in jobs), "adaptive": sum(j["hoption"] == 2 for j in jobs),
                    "search": sum(j["hoption"] == 3 for j in jobs)},
            "sequential_ms_by_kind": {k: float(sum(r[0]["run_ms"] for j, r in zip(jobs, seq) if j["hoption"] == h))

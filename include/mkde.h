@@ -331,6 +331,9 @@ typedef struct {
     int n_candidates;      /* MKDE_H_SEARCH: K <= 1024 bandwidths */
     float cand_lo, cand_hi; /* h_k = (float)(h0*lo*(hi/lo)^(k/(K-1))), h0 = rule of thumb */
     float *out;            /* nrows*ncols density raster (host) */
+    int one_candidate_per_pass; /* MKDE_H_SEARCH: 0 = up to 16 candidates per pass over the
+                                   point pairs (batched); 1 = one pass per candidate, as the
+                                   reference's per-candidate LogLikelihood calls (same result) */
 } mkde_serve_job;
 
 typedef struct {

@@ -106,6 +106,7 @@ class ServeJob(ctypes.Structure):
         ("cand_lo", ctypes.c_float),
         ("cand_hi", ctypes.c_float),
         ("out", ctypes.POINTER(ctypes.c_float)),
+        ("one_candidate_per_pass", ctypes.c_int),
     ]
 
 
@@ -510,7 +511,7 @@ class Server:
         self._keep = {}
 
     def submit(self, x, y, g, mask=None, out=None, w=None, h=None, hoption=0, edge_correction=True, optimise=False,
-               alpha=-0.5, n_candidates=16, cand_lo=0.1, cand_hi=10.0):
+               alpha=-0.5, n_candidates=16, cand_lo=0.1, cand_hi=10.0, one_candidate_per_pass=False):
         """Queue one job; returns its ticket.  The density lands in `out`
         (allocated here when None) once wait() reports it finished."""
         x, y = _f32(x), _f32(y)
@@ -522,7 +523,7 @@ class Server:
         assert out.dtype == np.float32 and out.flags.c_contiguous and out.size == g.ncols * g.nrows
         j = ServeJob(len(x), _p(x, _FP), _p(y, _FP), _p(w, _FP), _p(h, _FP), g, _p(m, _FP), int(hoption),
                      1 if edge_correction else 0, 1 if optimise else 0, float(alpha), int(n_candidates),
-                     float(cand_lo), float(cand_hi), _p(out, _FP))
+                     float(cand_lo), float(cand_hi), _p(out, _FP), 1 if one_candidate_per_pass else 0)
         t = ctypes.c_int64()
         _check(_lib.mkde_server_submit(self._h, ctypes.byref(j), ctypes.byref(t)))
         self._keep[t.value] = (x, y, w, h, m, out)

@@ -168,7 +168,11 @@ static int run_job(mkde_ctx *c, const mkde_serve_job &j, mkde_serve_result &r)
         const double lo = j.cand_lo, ratio = (double)j.cand_hi / (double)j.cand_lo;
         for (int k = 0; k < K; ++k)
             hc[k] = (float)((double)h0 * lo * std::pow(ratio, K > 1 ? (double)k / (double)(K - 1) : 0.0));
-        SCHK(mkde_loo_loglik(c, K, hc.data(), j.edge_correction, L.data()));
+        if (j.one_candidate_per_pass) {
+            for (int k = 0; k < K; ++k) SCHK(mkde_loo_loglik(c, 1, &hc[k], j.edge_correction, &L[k]));
+        } else {
+            SCHK(mkde_loo_loglik(c, K, hc.data(), j.edge_correction, L.data()));
+        }
         int best = 0;
         for (int k = 1; k < K; ++k)
             if (L[k] > L[best]) best = k;

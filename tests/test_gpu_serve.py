@@ -177,3 +177,22 @@ def test_footprint_bound_holds_for_each_job_kind():
         c.release()
         assert c.memory() == 0
         c.close()
+
+
+def test_one_candidate_per_pass_selects_the_same_bandwidth():
+    # the plain form (one leave-one-out pass per candidate) and the batched
+    # form (16 candidates per pass) select the same bandwidth; the rasters
+    # agree to the FP32 sum tolerance
+    x, y = cases.mixture_points(2500, 240.0, n_centres=10, seed=43)
+    mask = cases.disk_mask(240, 240, 110.0, n_holes=3, hole_min=5, hole_max=30, seed=43)
+    g = M.grid(240, 240, 0.0, 0.0, 1.0)
+    with M.Server(slots=2) as srv:
+        rb, ob = srv.wait(srv.submit(x, y, g, mask, hoption=M.MKDE_H_SEARCH, n_candidates=16))
+        rp, op = srv.wait(srv.submit(x, y, g, mask, hoption=M.MKDE_H_SEARCH, n_candidates=16,
+                                     one_candidate_per_pass=True))
+    assert rb["status"] == 0 and rp["status"] == 0
+    assert rb["h_sel"] == rp["h_sel"]
+    valid = mask != -9999.0
+    pos = valid & (ob > 0)
+    assert np.all(op[~valid] == ob[~valid])
+    assert np.max(np.abs(op[pos] - ob[pos]) / ob[pos]) <= 1e-5
