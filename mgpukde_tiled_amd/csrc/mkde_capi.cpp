@@ -51,8 +51,9 @@ hipError_t bt_build(hipStream_t, const GridDesc &, const uint8_t *, uint4 *, flo
 hipError_t dist_tree(hipStream_t, int64_t, const float *, const float *, const int *, const GridDesc &,
                      const BndTree &, double *);
 hipError_t ec_active(hipStream_t, int64_t, const double *, const float *, float, float *, int *, int *);
-hipError_t ec(hipStream_t, int, const GridDesc &, const MaskRuns &, const float *, const float *, const float *, float,
-              const int *, float *, unsigned long long *);
+hipError_t centres(hipStream_t, const GridDesc &, float *, float *);
+hipError_t ec(hipStream_t, int, const GridDesc &, const MaskRuns &, const float *, const float *, const float *,
+              const float *, const float *, float, const int *, float *, unsigned long long *);
 hipError_t count_pairs(hipStream_t, const GridDesc &, const MaskRuns &, const PointIndex &, unsigned long long *);
 hipError_t reach(hipStream_t, int, const double *, double *);
 hipError_t gml(hipStream_t, const double *, int64_t, double *);
@@ -176,6 +177,7 @@ struct mkde_ctx {
     DevBuf<uint8_t> valid;
     DevBuf<int64_t> run_ptr;
     DevBuf<int2> run_se;
+    DevBuf<float> colx, rowy;   // cell-centre tables of the mask grid
     DevBuf<uint8_t> bflags;
     DevBuf<float2> blist;
     DevBuf<unsigned long long> bcount;
@@ -216,6 +218,7 @@ struct mkde_ctx {
     // work list of the density kernel: heavy blocks split into parts,
     // items visited heaviest first
     bool dt_order_on = true;
+    int pts_per_bin = 32;   // target points per bin of the point index
     DevBuf<unsigned> blk_cost, blk_parts, blk_item0, blk_slot0, blk_sidx, blk_sidx2, blk_chunks, blk_counts,
         blk_hist;
     DevBuf<unsigned long long> blk_total;
@@ -256,6 +259,8 @@ extern "C" int mkde_ctx_create(int device, mkde_ctx **out)
     // tuning knob: MKDE_DENSITY_ORDER=0 visits density blocks in raster order
     const char *ord = getenv("MKDE_DENSITY_ORDER");
     if (ord && ord[0] == '0') c->dt_order_on = false;
+    const char *ppb = getenv("MKDE_PTS_PER_BIN");
+    if (ppb && atoi(ppb) > 0) c->pts_per_bin = atoi(ppb);
     e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
@@ -351,7 +356,7 @@ static std::vector<int> build_bins(mkde_ctx *c)
     }
     if (n == 0) { x0 = y0 = 0.0f; x1 = y1 = 1.0f; }
     double wx = std::max((double)x1 - x0, 1e-6), wy = std::max((double)y1 - y0, 1e-6);
-    double target = std::max(1.0, (double)n / 32.0);
+    double target = std::max(1.0, (double)n / (double)c->pts_per_bin);
     double size = std::sqrt(wx * wy / target);
     size = std::max(size, std::max(wx, wy) / 4096.0);
     int nbx, nby;
@@ -578,6 +583,9 @@ extern "C" int mkde_mask_upload(mkde_ctx *c, const mkde_grid *g, const float *va
         return fail(MKDE_E_NOMEM, "out of host memory");
     }
     HIPCHK(mkde_dev::valid_cells(c->stream, nc, c->mask.p, g->nodata, c->valid.p));
+    CHK(c->colx.ensure(g->ncols));
+    CHK(c->rowy.ensure(g->nrows));
+    HIPCHK(mkde_dev::centres(c->stream, c->mg, c->colx.p, c->rowy.p));
     c->mask_set = true;
     c->bflags_ok = false;
     c->nboundary = -1;
@@ -749,7 +757,8 @@ static int run_ec(mkde_ctx *c, const float *h, float hf, float *ew_out, unsigned
     CHK(copy_d2h(c, &nact, c->icount.p, sizeof nact));
     MaskRuns R{c->run_ptr.p, c->run_se.p};
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(mkde_dev::ec(c->stream, nact, c->mg, R, c->x.p, c->y.p, h, hf, c->list.p, ew_out, c->ucount.p));
+    HIPCHK(mkde_dev::ec(c->stream, nact, c->mg, R, c->colx.p, c->rowy.p, c->x.p, c->y.p, h, hf, c->list.p, ew_out,
+                        c->ucount.p));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     if (n_eval) CHK(copy_d2h(c, n_eval, c->ucount.p, sizeof *n_eval));
     HIPCHK(hipEventSynchronize(c->ev1));

@@ -1646,6 +1646,7 @@ __global__ __launch_bounds__(BT_WG) void k_dist_tree(int64_t n, const float *xs,
 struct EcArgs {
     GridDesc g;
     MaskRuns runs;
+    const float *colx, *rowy;   // cell-centre tables of the mask grid
     const float *x, *y;     // original order
     const float *h;         // original order (nullable when h_fixed > 0)
     float h_fixed;
@@ -1689,35 +1690,58 @@ __device__ __forceinline__ void narrow_box(const GridDesc &g, float px, float py
     cu = (int)(int32_t)(uint32_t)((nC - 1 < kr) ? nC - 1 : kr);
 }
 
-__device__ __forceinline__ bool pred_col(double xll, double cs, int64_t c, float px, double dy2, double thr)
+// Column centres of the mask grid: from a per-mask table (EC) or computed
+// (count pass on a tile); both give exactly cell_x.
+struct ColTable {
+    const float *x;
+    __device__ __forceinline__ float operator()(int64_t c) const { return x[c]; }
+};
+struct ColCompute {
+    double xll, cs;
+    __device__ __forceinline__ float operator()(int64_t c) const { return cell_x(xll, cs, c); }
+};
+
+template <class COL>
+__device__ __forceinline__ bool pred_col(const COL &colx, int64_t c, float px, double dy2, double thr)
 {
-    const float dx = cell_x(xll, cs, c) - px;
+    const float dx = colx(c) - px;
     return __dadd_rn(sqd(dx), dy2) < thr;
 }
 
 // Exact column interval [lo, hi] within [cl, cu] of the cells of one row that
 // are inside the cut-off; lo > hi when empty.  cn is the column of [cl, cu]
-// nearest to px (|dx| is smallest there).
-__device__ __forceinline__ void row_interval(double xll, double cs, int cl, int cu, int cn, float px, double dy2,
-                                             double thr, int &lo, int &hi)
+// nearest to px (|dx| is smallest there).  The first guess (FP32 square root,
+// multiplication by 1/cs) only seeds the search: the ends are then moved by
+// the exact predicate until they are the last columns inside.
+template <class COL>
+__device__ __forceinline__ void row_interval(const COL &colx, double xll, double inv_cs, int cl, int cu, int cn,
+                                             float px, double dy2, double thr, int &lo, int &hi)
 {
-    if (!pred_col(xll, cs, cn, px, dy2, thr)) {
+    if (!pred_col(colx, cn, px, dy2, thr)) {
         lo = 1;
         hi = 0;
         return;
     }
     const double rem = thr - dy2;
-    const double w = rem > 0.0 ? sqrt(rem) : 0.0;
-    double fh = floor(((double)px + w - xll) / cs - 0.5);
-    double fl = ceil(((double)px - w - xll) / cs - 0.5);
+    const double w = rem > 0.0 ? (double)sqrtf((float)rem) : 0.0;
+    const double fh = floor(((double)px + w - xll) * inv_cs - 0.5);
+    const double fl = ceil(((double)px - w - xll) * inv_cs - 0.5);
     int h_e = (int)fmin(fmax(fh, (double)cn), (double)cu);
     int l_e = (int)fmin(fmax(fl, (double)cl), (double)cn);
-    while (h_e < cu && pred_col(xll, cs, h_e + 1, px, dy2, thr)) ++h_e;
-    while (h_e > cn && !pred_col(xll, cs, h_e, px, dy2, thr)) --h_e;
-    while (l_e > cl && pred_col(xll, cs, l_e - 1, px, dy2, thr)) --l_e;
-    while (l_e < cn && !pred_col(xll, cs, l_e, px, dy2, thr)) ++l_e;
+    while (h_e < cu && pred_col(colx, h_e + 1, px, dy2, thr)) ++h_e;
+    while (h_e > cn && !pred_col(colx, h_e, px, dy2, thr)) --h_e;
+    while (l_e > cl && pred_col(colx, l_e - 1, px, dy2, thr)) --l_e;
+    while (l_e < cn && !pred_col(colx, l_e, px, dy2, thr)) ++l_e;
     lo = l_e;
     hi = h_e;
+}
+
+// Cell-centre tables of a grid: colx[c] = cell_x(c), rowy[r] = cell_y(r).
+__global__ void k_centres(GridDesc g, float *colx, float *rowy)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < g.ncols) colx[i] = cell_x(g.xll, g.cs, i);
+    if (i < g.nrows) rowy[i] = cell_y(g.yll, g.cs, (double)g.nrows - 0.5, i);
 }
 
 // First run index in [rs, re) whose end >= c (binary search).
@@ -1740,7 +1764,7 @@ __global__ __launch_bounds__(64) void k_ec(EcArgs A)
     const int lane = threadIdx.x;
     const int p = A.list[blockIdx.x];
     const GridDesc g = A.g;
-    const double xll = g.xll, yll = g.yll, cs = g.cs, nrm05 = (double)g.nrows - 0.5;
+    const double xll = g.xll, cs = g.cs, inv_cs = 1.0 / cs;
     const float px = A.x[p], py = A.y[p];
     const float hh = A.h_fixed > 0.0f ? A.h_fixed : A.h[p];
     const double h2 = (double)(float)(hh * hh);
@@ -1763,7 +1787,7 @@ __global__ __launch_bounds__(64) void k_ec(EcArgs A)
             for (int t = 0; t < per; ++t) {
                 const int k = my0 + t;
                 if (k < w) {
-                    const float dx = cell_x(xll, cs, ca + k) - px;
+                    const float dx = A.colx[ca + k] - px;
                     const float ex = fast_exp2((float)(sqd(dx) * a));
                     run = ff_add_f(run, ex);
                     s_ph[k] = run.hi;
@@ -1789,11 +1813,11 @@ __global__ __launch_bounds__(64) void k_ec(EcArgs A)
             __syncthreads();
             // rows
             for (int r = rl + lane; r <= ru; r += WAVE) {
-                const float cy = cell_y(yll, cs, nrm05, r);
+                const float cy = A.rowy[r];
                 const double dy2 = sqd(cy - py);
                 if (!(dy2 < thr)) continue;
                 int lo, hi;
-                row_interval(xll, cs, cl, cu, cn, px, dy2, thr, lo, hi);
+                row_interval(ColTable{A.colx}, xll, inv_cs, cl, cu, cn, px, dy2, thr, lo, hi);
                 if (lo < ca) lo = ca;
                 if (hi > cb) hi = cb;
                 if (lo > hi) continue;
@@ -1856,7 +1880,7 @@ __global__ __launch_bounds__(64) void k_count_pairs(CountArgs A)
                 const double dy2 = sqd(cy - P.y);
                 if (!(dy2 < P.thr)) continue;
                 int lo, hi;
-                row_interval(xll, cs, cl, cu, cn, P.x, dy2, P.thr, lo, hi);
+                row_interval(ColCompute{xll, cs}, xll, 1.0 / cs, cl, cu, cn, P.x, dy2, P.thr, lo, hi);
                 if (lo > hi) continue;
                 const int64_t rs = A.runs.run_ptr[r], re = A.runs.run_ptr[r + 1];
                 for (int64_t k = first_run_ending_at_or_after(A.runs.run_se, rs, re, lo); k < re; ++k) {
@@ -2192,10 +2216,18 @@ hipError_t ec_active(hipStream_t s, int64_t n, const double *dist, const float *
     return hipGetLastError();
 }
 
-hipError_t ec(hipStream_t s, int nactive, const GridDesc &g, const MaskRuns &runs, const float *x, const float *y,
-              const float *h, float h_fixed, const int *list, float *ew, unsigned long long *n_eval)
+hipError_t centres(hipStream_t s, const GridDesc &g, float *colx, float *rowy)
 {
-    EcArgs A{g, runs, x, y, h, h_fixed, list, ew, n_eval};
+    const int64_t m = g.ncols > g.nrows ? g.ncols : g.nrows;
+    hipLaunchKernelGGL(k_centres, dim3(nblk(m, 256)), dim3(256), 0, s, g, colx, rowy);
+    return hipGetLastError();
+}
+
+hipError_t ec(hipStream_t s, int nactive, const GridDesc &g, const MaskRuns &runs, const float *colx,
+              const float *rowy, const float *x, const float *y, const float *h, float h_fixed, const int *list,
+              float *ew, unsigned long long *n_eval)
+{
+    EcArgs A{g, runs, colx, rowy, x, y, h, h_fixed, list, ew, n_eval};
     if (nactive > 0) hipLaunchKernelGGL(k_ec, dim3((unsigned)nactive), dim3(64), 0, s, A);
     return hipGetLastError();
 }
