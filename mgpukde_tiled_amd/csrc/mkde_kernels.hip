@@ -32,6 +32,7 @@
 #include <limits.h>
 #include <algorithm>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "mkde_device.h"
 
@@ -1635,13 +1636,12 @@ __global__ __launch_bounds__(BT_WG) void k_dist_tree(int64_t n, const float *xs,
 // and for a fixed row the cells inside the cut-off form one interval of
 // columns (the reference's test is monotone in |dx|).  One wave handles one
 // point: columns of the NARROW box (tiled:2716-2727) are processed in chunks
-// of EC_CW, with a float-float prefix sum of exp2(dx2 a) in LDS; lanes then
+// of EC_CW (1024), with a float-float prefix sum of exp2(dx2 a) in LDS; lanes then
 // take rows, find each row's exact column interval, intersect it with the
 // row's valid-cell runs and add exp2(dy2 a) * (prefix differences).
 // Output ew = 1/(cellArea * sum / (2 pi h2f)); points whose boundary distance
 // is >= 9 h2f were given weight 1 by k_ec_active.
 
-#define EC_CW 2048
 
 struct EcArgs {
     GridDesc g;
@@ -1756,6 +1756,7 @@ __device__ __forceinline__ int64_t first_run_ending_at_or_after(const int2 *runs
     return lo;
 }
 
+template <int EC_CW>
 __global__ __launch_bounds__(64) void k_ec(EcArgs A)
 {
     __shared__ float s_ph[EC_CW];
@@ -2228,7 +2229,18 @@ hipError_t ec(hipStream_t s, int nactive, const GridDesc &g, const MaskRuns &run
               float *ew, unsigned long long *n_eval)
 {
     EcArgs A{g, runs, colx, rowy, x, y, h, h_fixed, list, ew, n_eval};
-    if (nactive > 0) hipLaunchKernelGGL(k_ec, dim3((unsigned)nactive), dim3(64), 0, s, A);
+    // column chunk of the prefix sums (LDS 8 B per column and wave); 1024
+    // measured best (512 / 1024 / 2048: the adaptive job's EC within 1%, the
+    // 64-candidate search 314 / 296 / 307 ms); MKDE_EC_CHUNK overrides it
+    static const int cw = [] {
+        const char *e = getenv("MKDE_EC_CHUNK");
+        return e ? atoi(e) : 1024;
+    }();
+    if (nactive > 0) {
+        if (cw == 512) hipLaunchKernelGGL(k_ec<512>, dim3((unsigned)nactive), dim3(64), 0, s, A);
+        else if (cw == 1024) hipLaunchKernelGGL(k_ec<1024>, dim3((unsigned)nactive), dim3(64), 0, s, A);
+        else hipLaunchKernelGGL(k_ec<2048>, dim3((unsigned)nactive), dim3(64), 0, s, A);
+    }
     return hipGetLastError();
 }
 
