@@ -261,28 +261,42 @@ __device__ __forceinline__ void fma16_bcast(float (&a)[16], float eyv, float ex)
         : "v"(eyv), "v"(ex));
 }
 
-// Rows R..R+3 of a PARTIAL sub-block: m_i = fl64(dx2 + dy2_i) < thr ? ex : 0
-// (the exact cut-off), then acc[R+i] += Ey[R+i] * m_i.  The four masks are
+// Reach of one raster row for one point: the smallest float f >= 0 with
+// fl64(f^2 + dy2) >= thr.  f^2 is exact in FP64 and the sum is monotone in f,
+// so a cell of the row is inside the cut-off fl64(dx2 + dy2) < thr exactly
+// when |fl32(cx - px)| < f; f = 0 when no cell of the row is inside.  The
+// hardware FP64 square root (within about one FP32 ulp) only seeds the search;
+// the exact predicate then moves f by single ulps.
+__device__ __forceinline__ float row_reach(double dy2, double thr)
+{
+    if (!(dy2 < thr)) return 0.0f;
+    auto inside = [&](float v) { return __dadd_rn((double)v * (double)v, dy2) < thr; };
+    float f = (float)__builtin_amdgcn_sqrt(thr - dy2);
+    for (int k = 0; k < 4096 && inside(f); ++k) f = __uint_as_float(__float_as_uint(f) + 1u);
+    for (int k = 0; k < 4096 && f > 0.0f && !inside(__uint_as_float(__float_as_uint(f) - 1u)); ++k)
+        f = __uint_as_float(__float_as_uint(f) - 1u);
+    return f;
+}
+
+// Rows R..R+3 of a PARTIAL sub-block: m_i = |dx| < reach_i ? ex : 0 (the
+// exact cut-off, row_reach), then acc[R+i] += Ey[R+i] * m_i with Ey broadcast
+// by DPP.  The four lane masks go to four SGPR pairs and the four m_i are
 // formed before the four DPP FMAs, so each m_i is at least 2 VALU
 // instructions old when it is read.
-#define MKDE_PART_ROW(I) \
-    "v_add_f64 %[t], %[dx2], %[d" #I "]\n"   \
-    "v_cmp_lt_f64 vcc, %[t], %[thr]\n"        \
-    "v_cndmask_b32 %[m" #I "], 0, %[ex], vcc\n"
-#define MKDE_PART_FMA(I, J) "v_fmac_f32_dpp %[a" #I "], %[ey], %[m" #I "] row_newbcast:" #J " row_mask:0xf bank_mask:0xf\n"
+#define MKDE_XCMP(I) "v_cmp_gt_f32_e64 %[s" #I "], %[x" #I "], |%[dx]|\n"
+#define MKDE_XSEL(I) "v_cndmask_b32_e64 %[m" #I "], 0, %[ex], %[s" #I "]\n"
+#define MKDE_XFMA(I, J) "v_fmac_f32_dpp %[a" #I "], %[ey], %[m" #I "] row_newbcast:" #J " row_mask:0xf bank_mask:0xf\n"
 #define MKDE_PART4(R0, R1, R2, R3)                                                                                    \
-    __device__ __forceinline__ void part4_##R0(float &a0, float &a1, float &a2, float &a3, double dx2, double d0,      \
-                                               double d1, double d2, double d3, double thr, float ex, float eyv)      \
+    __device__ __forceinline__ void part4_##R0(float &a0, float &a1, float &a2, float &a3, float4 x, float ex,         \
+                                               float eyv, float dx)                                                   \
     {                                                                                                                 \
         float m0, m1, m2, m3;                                                                                         \
-        double t;                                                                                                     \
-        asm(MKDE_PART_ROW(0) MKDE_PART_ROW(1) MKDE_PART_ROW(2) MKDE_PART_ROW(3) MKDE_PART_FMA(0, R0)                 \
-                MKDE_PART_FMA(1, R1) MKDE_PART_FMA(2, R2) MKDE_PART_FMA(3, R3)                                        \
+        uint64_t s0, s1, s2, s3;                                                                                      \
+        asm(MKDE_XCMP(0) MKDE_XCMP(1) MKDE_XCMP(2) MKDE_XCMP(3) MKDE_XSEL(0) MKDE_XSEL(1) MKDE_XSEL(2) MKDE_XSEL(3)   \
+                MKDE_XFMA(0, R0) MKDE_XFMA(1, R1) MKDE_XFMA(2, R2) MKDE_XFMA(3, R3)                                   \
             : [a0] "+v"(a0), [a1] "+v"(a1), [a2] "+v"(a2), [a3] "+v"(a3), [m0] "=&v"(m0), [m1] "=&v"(m1),             \
-              [m2] "=&v"(m2), [m3] "=&v"(m3), [t] "=&v"(t)                                                            \
-            : [dx2] "v"(dx2), [d0] "v"(d0), [d1] "v"(d1), [d2] "v"(d2), [d3] "v"(d3), [thr] "v"(thr), [ex] "v"(ex),   \
-              [ey] "v"(eyv)                                                                                           \
-            : "vcc");                                                                                                 \
+              [m2] "=&v"(m2), [m3] "=&v"(m3), [s0] "=&s"(s0), [s1] "=&s"(s1), [s2] "=&s"(s2), [s3] "=&s"(s3)          \
+            : [x0] "v"(x.x), [x1] "v"(x.y), [x2] "v"(x.z), [x3] "v"(x.w), [ex] "v"(ex), [ey] "v"(eyv), [dx] "v"(dx));   \
     }
 MKDE_PART4(0, 1, 2, 3)
 MKDE_PART4(4, 5, 6, 7)
@@ -299,19 +313,20 @@ MKDE_PART4(12, 13, 14, 15)
 // every cell inside; PARTIAL), queues the non-empty ones in bin order, and
 // consumes them in sub-batches of DT_SB points:
 //   1. cooperatively, in LDS: Ex[p][col] = exp2(dx2 a_p) for the 64 columns,
-//      Ey[p][row] = coef_p exp2(dy2 a_p) and dy2 for the 64 rows, and for
-//      PARTIAL points the class of each wave's 64x16 sub-block (the same exact
-//      EMPTY / FULL test on the smaller rectangle);
+//      Ey[p][row] = coef_p exp2(dy2 a_p) for the 64 rows, and for PARTIAL
+//      points the exact reach of each row (row_reach) and the class of each
+//      wave's 64x16 sub-block (the same exact EMPTY / FULL test on the
+//      smaller rectangle);
 //   2. each wave adds Ex * Ey[p][row] to its 16 cells with one FMA each
-//      (FULL), or with the exact cut-off test fl64(dx2 + dy2) < thr first
-//      (PARTIAL), and skips EMPTY sub-blocks;
+//      (FULL), or with Ex replaced by 0 on the lanes with |dx| >= reach[p][row]
+//      (PARTIAL: the exact cut-off fl64(dx2 + dy2) < thr), and skips EMPTY
+//      sub-blocks;
 //   3. the FP32 partial sums are Kahan-folded after every sub-batch.
 // The order of points is fixed (bins in row-major order, points in bin order)
 // so results are deterministic.
 
 #define DT_SB 32
-#define DT_EYP 68   // floats per point row of the Ey table (64 + pad)
-#define DT_DYP 66   // doubles per point row of the dy2 table (64 + pad)
+#define DT_EYP 68   // floats per point row of the Ey and reach tables (64 + pad)
 #define DT_Q (DT_SB + 256)
 
 // stats[0] candidate points classified, [1] FULL blocks, [2] PARTIAL blocks,
@@ -402,7 +417,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     __shared__ int s_nfull;         // block-FULL points at the front of the sub-batch
     __shared__ __attribute__((aligned(16))) float s_ex[DT_SB * 64];
     __shared__ __attribute__((aligned(16))) float s_ey[DT_SB * DT_EYP];
-    __shared__ __attribute__((aligned(16))) double s_dy2[DT_SB * DT_DYP];
+    __shared__ __attribute__((aligned(16))) float s_xr[DT_SB * DT_EYP];   // row reach, PARTIAL points
     __shared__ float s_ccx[64], s_ccy[64];   // cell-centre x of the block's columns, y of its rows
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -514,6 +529,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         __syncthreads();
         // 1b. tables: lane -> row / column slot, wave -> 8 points; consecutive
         //     lanes write consecutive LDS words (no bank conflicts)
+        const int nfull = s_nfull;
         {
             const float cyr = s_ccy[lane], cxr = s_ccx[lane];
 #pragma unroll 2
@@ -523,9 +539,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                     const float py = s_py[p], px = s_px[p], coef = s_coef[p];
                     const double a = s_a[p];
                     const double dy2 = sqd(cyr - py);
-                    s_dy2[p * DT_DYP + lane] = dy2;
                     s_ey[p * DT_EYP + lane] = fast_exp2((float)(dy2 * a)) * coef;
                     s_ex[p * 64 + lane] = fast_exp2((float)(sqd(cxr - px) * a));
+                    if (p >= nfull) s_xr[p * DT_EYP + lane] = row_reach(dy2, s_thr[p]);
                 }
             }
         }
@@ -533,7 +549,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         // 2a. block-FULL points: one FMA per cell, no tests.  Ex of this
         //     lane's column from LDS, Ey of the wave's 16 rows as one VGPR
         //     (lane l: row l & 15) broadcast by DPP.
-        const int nfull = s_nfull;
         const int eyo = wave * 16 + (lane & 15);
         {
             // two points per step: the second pair of loads is in flight
@@ -564,20 +579,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 fma16_bcast(acc, eyv, ex);
                 if (STATS && lane == 0) ++st_wf;
             } else {
-                const double thr = s_thr[p];
-                const double dx2 = sqd(cx - s_px[p]);
-                const double2 *dyp = (const double2 *)&s_dy2[p * DT_DYP + wave * 16];
-                double2 u = dyp[0], v = dyp[1];
-                part4_0(acc[0], acc[1], acc[2], acc[3], dx2, u.x, u.y, v.x, v.y, thr, ex, eyv);
-                u = dyp[2];
-                v = dyp[3];
-                part4_4(acc[4], acc[5], acc[6], acc[7], dx2, u.x, u.y, v.x, v.y, thr, ex, eyv);
-                u = dyp[4];
-                v = dyp[5];
-                part4_8(acc[8], acc[9], acc[10], acc[11], dx2, u.x, u.y, v.x, v.y, thr, ex, eyv);
-                u = dyp[6];
-                v = dyp[7];
-                part4_12(acc[12], acc[13], acc[14], acc[15], dx2, u.x, u.y, v.x, v.y, thr, ex, eyv);
+                const float dx = cx - s_px[p];
+                const float4 *xr = (const float4 *)&s_xr[p * DT_EYP + wave * 16];
+                part4_0(acc[0], acc[1], acc[2], acc[3], xr[0], ex, eyv, dx);
+                part4_4(acc[4], acc[5], acc[6], acc[7], xr[1], ex, eyv, dx);
+                part4_8(acc[8], acc[9], acc[10], acc[11], xr[2], ex, eyv, dx);
+                part4_12(acc[12], acc[13], acc[14], acc[15], xr[3], ex, eyv, dx);
                 if (STATS && lane == 0) ++st_wp;
             }
         }
