@@ -321,13 +321,16 @@ MKDE_PART4(12, 13, 14, 15)
 //      (FULL), or with Ex replaced by 0 on the lanes with |dx| >= reach[p][row]
 //      (PARTIAL: the exact cut-off fl64(dx2 + dy2) < thr), and skips EMPTY
 //      sub-blocks;
-//   3. the FP32 partial sums are Kahan-folded after every sub-batch.
+//   3. the FP32 partial sums are Kahan-folded after every DT_FOLD sub-batches.
 // The order of points is fixed (bins in row-major order, points in bin order)
 // so results are deterministic.
 
 #define DT_SB 32
 #define DT_EYP 68   // floats per point row of the Ey and reach tables (64 + pad)
 #define DT_Q (DT_SB + 256)
+// Sub-batches summed in FP32 before a Kahan fold: at most 64 positive terms,
+// a relative error below 64 * 2^-24 = 3.8e-6 per fold
+#define DT_FOLD 2
 
 // stats[0] candidate points classified, [1] FULL blocks, [2] PARTIAL blocks,
 // [3] FULL / [4] PARTIAL / [5] EMPTY wave sub-blocks of PARTIAL points,
@@ -485,6 +488,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int nb_total = nbx_r * (byb - bya + 1);
 
     int qn = 0;       // queue length (uniform)
+    int npend = 0;    // sub-batches in acc since the last fold (uniform)
+    auto fold = [&]() {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            kahan_fold(acc[j], sum[j], comp[j]);
+            acc[j] = 0.0f;
+        }
+        npend = 0;
+    };
     int64_t ord = 0;  // candidate ordinal at the start of the next candidate bin (uniform)
 
     auto process = [&](int cnt) {
@@ -588,11 +600,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 if (STATS && lane == 0) ++st_wp;
             }
         }
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            kahan_fold(acc[j], sum[j], comp[j]);
-            acc[j] = 0.0f;
-        }
+        if (++npend == DT_FOLD) fold();
         __syncthreads();
     };
 
@@ -670,6 +678,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         __syncthreads();
     }
     if (qn > 0) process(qn);
+    if (npend > 0) fold();
 
     // write back: nodata cells keep the mask value; parts of a split block
     // leave FP64 partials (sum - compensation) for k_density_merge
