@@ -1424,9 +1424,16 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
         c->pts_dirty = true;
         return s;
     };
-    if (ec_on) CHK(ec_pass());
-    else HIPCHK(mkde_dev::fill_f32(c->stream, c->n, 1.0f, c->ew.p));
-    c->pts_dirty = true;
+    // EC weights at the job's final bandwidths.  The bandwidth searches
+    // (hoption 1 and 2) compute their own EC per candidate and never read the
+    // context weights, so those jobs run the final EC pass only.
+    auto final_ec = [&](void) -> int {
+        if (ec_on) return ec_pass();
+        HIPCHK(mkde_dev::fill_f32(c->stream, c->n, 1.0f, c->ew.p));
+        c->pts_dirty = true;
+        return MKDE_OK;
+    };
+    if (o->hoption == 0 || o->hoption == -1) CHK(final_ec());
     if (o->hoption == 1) {
         {
             PhaseTimer pt(c->stream, &S.ms_bandwidth);
@@ -1434,7 +1441,7 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
             CHK(mkde_mle_fixed_bandwidth(c, S.h0, ec_on, &S.h_opt, &it));
         }
         CHK(mkde_set_bandwidth_fixed(c, S.h_opt));
-        if (ec_on) CHK(ec_pass());
+        CHK(final_ec());
     } else if (o->hoption == 2) {
         float h0 = S.h0, a0 = o->optimise ? -0.5f : o->alpha;
         {
@@ -1452,7 +1459,7 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
         S.h_opt = h0;
         S.alpha_opt = a0;
         CHK(copy_d2h(c, &S.g, c->gval.p, sizeof(double)));
-        if (ec_on) CHK(ec_pass());
+        CHK(final_ec());
     }
     S.n_ec_pairs = (int64_t)ne;
     {
