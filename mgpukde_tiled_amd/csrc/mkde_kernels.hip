@@ -1726,28 +1726,39 @@ __device__ __forceinline__ bool pred_col(const COL &colx, int64_t c, float px, d
 
 // Exact column interval [lo, hi] within [cl, cu] of the cells of one row that
 // are inside the cut-off; lo > hi when empty.  cn is the column of [cl, cu]
-// nearest to px (|dx| is smallest there).  The first guess (FP32 square root,
-// multiplication by 1/cs) only seeds the search: the ends are then moved by
-// the exact predicate until they are the last columns inside.
+// nearest to px (|dx| is smallest there, and the predicate is monotone in the
+// distance from cn on either side).  The first guess (FP32 square root,
+// multiplication by 1/cs) only seeds the search: the five predicates that
+// confirm it (cn, both ends, one step beyond each end) are evaluated together,
+// so their centre loads are in flight at once; when the seed is off, the ends
+// are moved by the exact predicate until they are the last columns inside.
 template <class COL>
 __device__ __forceinline__ void row_interval(const COL &colx, double xll, double inv_cs, int cl, int cu, int cn,
                                              float px, double dy2, double thr, int &lo, int &hi)
 {
-    if (!pred_col(colx, cn, px, dy2, thr)) {
-        lo = 1;
-        hi = 0;
-        return;
-    }
     const double rem = thr - dy2;
     const double w = rem > 0.0 ? (double)sqrtf((float)rem) : 0.0;
     const double fh = floor(((double)px + w - xll) * inv_cs - 0.5);
     const double fl = ceil(((double)px - w - xll) * inv_cs - 0.5);
     int h_e = (int)fmin(fmax(fh, (double)cn), (double)cu);
     int l_e = (int)fmin(fmax(fl, (double)cl), (double)cn);
-    while (h_e < cu && pred_col(colx, h_e + 1, px, dy2, thr)) ++h_e;
-    while (h_e > cn && !pred_col(colx, h_e, px, dy2, thr)) --h_e;
-    while (l_e > cl && pred_col(colx, l_e - 1, px, dy2, thr)) --l_e;
-    while (l_e < cn && !pred_col(colx, l_e, px, dy2, thr)) ++l_e;
+    const int h1 = h_e < cu ? h_e + 1 : h_e, l1 = l_e > cl ? l_e - 1 : l_e;
+    const bool p_n = pred_col(colx, cn, px, dy2, thr);
+    const bool p_h = pred_col(colx, h_e, px, dy2, thr);
+    const bool p_h1 = pred_col(colx, h1, px, dy2, thr) & (h_e < cu);
+    const bool p_l = pred_col(colx, l_e, px, dy2, thr);
+    const bool p_l1 = pred_col(colx, l1, px, dy2, thr) & (l_e > cl);
+    if (!p_n) {
+        lo = 1;
+        hi = 0;
+        return;
+    }
+    if (!(p_h & !p_h1 & p_l & !p_l1)) {
+        while (h_e < cu && pred_col(colx, h_e + 1, px, dy2, thr)) ++h_e;
+        while (h_e > cn && !pred_col(colx, h_e, px, dy2, thr)) --h_e;
+        while (l_e > cl && pred_col(colx, l_e - 1, px, dy2, thr)) --l_e;
+        while (l_e < cn && !pred_col(colx, l_e, px, dy2, thr)) ++l_e;
+    }
     lo = l_e;
     hi = h_e;
 }
@@ -1833,12 +1844,13 @@ __global__ __launch_bounds__(64) void k_ec(EcArgs A)
                 const float cy = A.rowy[r];
                 const double dy2 = sqd(cy - py);
                 if (!(dy2 < thr)) continue;
+                // the row's run range is loaded alongside the interval search
+                const int64_t rs = A.runs.run_ptr[r], re = A.runs.run_ptr[r + 1];
                 int lo, hi;
                 row_interval(ColTable{A.colx}, xll, inv_cs, cl, cu, cn, px, dy2, thr, lo, hi);
                 if (lo < ca) lo = ca;
                 if (hi > cb) hi = cb;
                 if (lo > hi) continue;
-                const int64_t rs = A.runs.run_ptr[r], re = A.runs.run_ptr[r + 1];
                 ff seg = {0.0f, 0.0f};
                 for (int64_t k = first_run_ending_at_or_after(A.runs.run_se, rs, re, lo); k < re; ++k) {
                     const int2 rr = A.runs.run_se[k];
