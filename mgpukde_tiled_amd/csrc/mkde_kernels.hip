@@ -991,6 +991,11 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 __device__ __forceinline__ float uni(float v) { return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v))); }
+__device__ __forceinline__ int4 uniform_item(int4 v)
+{
+    return make_int4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
+                     __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
+}
 
 // Work estimate of each 64-query wave of the point kernels: the points in
 // its candidate bins (same culling as the kernels: bins within the reach of
@@ -1036,9 +1041,11 @@ __global__ __launch_bounds__(256) void k_point_density_w(PtsArgs A)
 {
     const PointIndex &I = A.idx;
     const int lane = threadIdx.x & 63;
-    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the wave index and the work item are wave-uniform: made explicit so the
+    // source records below are read with scalar loads
+    const int64_t g = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (g >= (A.order ? (int64_t)A.n_items : (I.n + 63) / 64)) return;   // whole wave
-    const int4 item = A.order ? A.items[A.order[g]] : make_int4((int)g, 0, 1, -1);
+    const int4 item = uniform_item(A.order ? A.items[A.order[g]] : make_int4((int)g, 0, 1, -1));
     int win0 = 0, win1 = INT_MAX;   // candidate counts are below 2^31
     if (item.z > 1) {
         const int64_t C = A.cost[item.x];
@@ -1205,9 +1212,9 @@ __global__ __launch_bounds__(256) void k_loo_multi_w(LooMultiArgs A)
     const int lane = threadIdx.x & 63;
     const int K = A.K;
     const int64_t n = I.n;
-    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t g = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (g >= (A.order ? (int64_t)A.n_items : (n + 63) / 64)) return;
-    const int4 item = A.order ? A.items[A.order[g]] : make_int4((int)g, 0, 1, -1);
+    const int4 item = uniform_item(A.order ? A.items[A.order[g]] : make_int4((int)g, 0, 1, -1));
     int win0 = 0, win1 = INT_MAX;   // candidate counts are below 2^31
     if (item.z > 1) {
         const int64_t C = A.cost[item.x];
