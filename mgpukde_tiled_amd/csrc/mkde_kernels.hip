@@ -410,7 +410,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 {
     __shared__ int s_bins[256];
     __shared__ int s_wcnt[4];
-    __shared__ unsigned s_queue[DT_Q];
+    __shared__ unsigned s_queue[DT_Q];   // point index | 0x80000000 for block-PARTIAL
+    // records of the queued points, stored when they are classified, so a
+    // sub-batch reads its parameters from LDS instead of global memory
+    __shared__ float s_qx[DT_Q], s_qy[DT_Q], s_qc[DT_Q];
+    __shared__ double s_qa[DT_Q], s_qt[DT_Q];
     __shared__ float s_px[DT_SB];
     __shared__ float s_py[DT_SB];
     __shared__ float s_coef[DT_SB];
@@ -422,6 +426,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     __shared__ __attribute__((aligned(16))) float s_ey[DT_SB * DT_EYP];
     __shared__ __attribute__((aligned(16))) float s_xr[DT_SB * DT_EYP];   // row reach, PARTIAL points
     __shared__ float s_ccx[64], s_ccy[64];   // cell-centre x of the block's columns, y of its rows
+    __shared__ float s_wyb[8];               // y of the first / last row of each wave's sub-block
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const GridDesc g = A.g;
@@ -475,6 +480,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const float cx = cell_x(xll, cs, lc0 + lane);
     if (tid < 64) s_ccx[tid] = cx;
     else if (tid < 128) s_ccy[tid - 64] = cell_y(yll, cs, nrm05, lr0 + (tid - 64));
+    else if (tid < 136) {
+        // wave w = (tid - 128) / 2: [0] its first row, [1] its last row
+        const int64_t wr0 = lr0 + 8 * (tid - 128 - (tid & 1));
+        const int64_t wr1 = (wr0 + 15 < lr1) ? wr0 + 15 : lr1;
+        s_wyb[tid - 128] = (tid & 1) ? cell_y(yll, cs, nrm05, wr1) : cell_y(yll, cs, nrm05, wr0);
+    }
 
     float acc[16], sum[16], comp[16];
 #pragma unroll
@@ -499,11 +510,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     };
     int64_t ord = 0;  // candidate ordinal at the start of the next candidate bin (uniform)
 
-    auto process = [&](int cnt) {
-        // 1a. point parameters, block-FULL points first (stable partition, so
-        //     the order stays deterministic); cnt <= DT_SB <= 64: one wave
+    auto process = [&](int cnt, int head) {
+        // 1a. point parameters of queue entries [head, head + cnt), block-FULL
+        //     points first (stable partition, so the order stays
+        //     deterministic); cnt <= DT_SB <= 64: one wave
         if (tid < 64) {
-            const unsigned q = tid < cnt ? s_queue[tid] : 0u;
+            const int e = head + tid;
+            const unsigned q = tid < cnt ? s_queue[e] : 0u;
             const bool isfull = tid < cnt && !(q >> 31);
             const uint64_t fm = __ballot(isfull);
             const uint64_t below = (1ull << tid) - 1ull;
@@ -511,31 +524,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             const int slot = isfull ? __popcll(fm & below) : nf + (tid - __popcll(fm & below));
             if (tid == 0) s_nfull = nf;
             if (tid < cnt) {
-            const PtRec p = I.pts[q & 0x7fffffffu];
-            const int t = slot;
-            s_px[t] = p.x;
-            s_py[t] = p.y;
-            s_coef[t] = p.coef;
-            s_a[t] = p.a;
-            s_thr[t] = p.thr;
-            // wave sub-block classes
-            int cl4 = 0x55;   // 2 bits per wave: 0 skip, 1 full, 2 partial
-            if (q >> 31) {
-                cl4 = 0;
-                const float fx = far1(p.x, bx0, bx1), nx = gap1(p.x, bx0, bx1);
+                const float px = s_qx[e], py = s_qy[e];
+                const double thr = s_qt[e];
+                const int t = slot;
+                s_px[t] = px;
+                s_py[t] = py;
+                s_coef[t] = s_qc[e];
+                s_a[t] = s_qa[e];
+                s_thr[t] = thr;
+                // wave sub-block classes
+                int cl4 = 0x55;   // 2 bits per wave: 0 skip, 1 full, 2 partial
+                if (q >> 31) {
+                    cl4 = 0;
+                    const float fx = far1(px, bx0, bx1), nx = gap1(px, bx0, bx1);
 #pragma unroll
-                for (int w = 0; w < 4; ++w) {
-                    const int64_t wr0 = lr0 + 16 * w;
-                    const int64_t wr1 = (wr0 + 15 < lr1) ? wr0 + 15 : lr1;
-                    int cw = 0;
-                    if (wr0 <= lr1) {
-                        const float wy1 = cell_y(yll, cs, nrm05, wr0), wy0 = cell_y(yll, cs, nrm05, wr1);
-                        if (sq2(nx, gap1(p.y, wy0, wy1)) < p.thr) cw = sq2(fx, far1(p.y, wy0, wy1)) < p.thr ? 1 : 2;
+                    for (int w = 0; w < 4; ++w) {
+                        int cw = 0;
+                        if (lr0 + 16 * w <= lr1) {
+                            const float wy1 = s_wyb[2 * w], wy0 = s_wyb[2 * w + 1];
+                            if (sq2(nx, gap1(py, wy0, wy1)) < thr) cw = sq2(fx, far1(py, wy0, wy1)) < thr ? 1 : 2;
+                        }
+                        cl4 |= cw << (2 * w);
                     }
-                    cl4 |= cw << (2 * w);
                 }
-            }
-            s_wcls[t] = cl4;
+                s_wcls[t] = cl4;
             }
         }
         __syncthreads();
@@ -544,9 +556,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         const int nfull = s_nfull;
         {
             const float cyr = s_ccy[lane], cxr = s_ccx[lane];
+            // the point index is wave-uniform: made explicit (scalar index
+            // and bound test, fewer VALU address operations)
+            const int pw = __builtin_amdgcn_readfirstlane(wave) * (DT_SB / 4);
 #pragma unroll 2
             for (int q = 0; q < DT_SB / 4; ++q) {
-                const int p = wave * (DT_SB / 4) + q;   // wave-uniform
+                const int p = pw + q;
                 if (p < cnt) {
                     const float py = s_py[p], px = s_px[p], coef = s_coef[p];
                     const double a = s_a[p];
@@ -632,8 +647,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             for (int off = st; off < en; off += 256) {
                 const int i = off + tid;
                 int cls = 0;   // 0 empty, 1 full, 2 partial
+                PtRec p;
                 if (i < en) {
-                    const PtRec p = I.pts[i];
+                    p = I.pts[i];
                     if (STATS) ++st_cand;
                     if (p.thr > 0.0) {
                         double near2 = sq2(gap1(p.x, bx0, bx1), gap1(p.y, by0, by1));
@@ -647,29 +663,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 }
                 int na;
                 int pos = block_compact256(cls != 0, s_wcnt, &na);
-                if (cls != 0) s_queue[qn + pos] = (unsigned)i | (cls == 2 ? 0x80000000u : 0u);
+                if (cls != 0) {
+                    const int e = qn + pos;
+                    s_queue[e] = (unsigned)i | (cls == 2 ? 0x80000000u : 0u);
+                    s_qx[e] = p.x;
+                    s_qy[e] = p.y;
+                    s_qc[e] = p.coef;
+                    s_qa[e] = p.a;
+                    s_qt[e] = p.thr;
+                }
                 __syncthreads();
                 qn += na;
                 if (qn >= DT_SB) {
                     int head = 0;
                     while (qn - head >= DT_SB) {
-                        if (head > 0) {
-                            // move the next sub-batch to the queue front
-                            unsigned v = 0;
-                            if (tid < DT_SB) v = s_queue[head + tid];
-                            __syncthreads();
-                            if (tid < DT_SB) s_queue[tid] = v;
-                            __syncthreads();
-                        }
-                        process(DT_SB);
+                        process(DT_SB, head);
                         head += DT_SB;
                     }
                     // keep the remainder at the front
                     const int rem = qn - head;
                     unsigned v = 0;
-                    if (tid < rem) v = s_queue[head + tid];
+                    float vx = 0.0f, vy = 0.0f, vc = 0.0f;
+                    double va = 0.0, vt = 0.0;
+                    if (tid < rem) {
+                        const int e = head + tid;
+                        v = s_queue[e];
+                        vx = s_qx[e];
+                        vy = s_qy[e];
+                        vc = s_qc[e];
+                        va = s_qa[e];
+                        vt = s_qt[e];
+                    }
                     __syncthreads();
-                    if (tid < rem) s_queue[tid] = v;
+                    if (tid < rem) {
+                        s_queue[tid] = v;
+                        s_qx[tid] = vx;
+                        s_qy[tid] = vy;
+                        s_qc[tid] = vc;
+                        s_qa[tid] = va;
+                        s_qt[tid] = vt;
+                    }
                     __syncthreads();
                     qn = rem;
                 }
@@ -677,7 +710,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         }
         __syncthreads();
     }
-    if (qn > 0) process(qn);
+    if (qn > 0) process(qn, 0);
     if (npend > 0) fold();
 
     // write back: nodata cells keep the mask value; parts of a split block
