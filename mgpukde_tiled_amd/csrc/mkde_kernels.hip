@@ -25,8 +25,9 @@
 //                exp2(fl32(d^2 * a)) with a = -log2(e)/(2 h2f); the density and
 //                edge-correction integrands use the separable form
 //                exp2(dx^2 a) * exp2(dy^2 a) over raster cells.
-//   sums         FP32 with compensation (Kahan folds of short FP32 partial sums,
-//                float-float prefix sums); FP64 only for log-likelihood sums.
+//   sums         short FP32 partial sums folded with compensation (Kahan) or
+//                into FP64; FP64 prefix sums (edge correction) and FP64
+//                log-likelihood sums.
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include <limits.h>
@@ -1692,7 +1693,7 @@ __global__ __launch_bounds__(BT_WG) void k_dist_tree(int64_t n, const float *xs,
 // and for a fixed row the cells inside the cut-off form one interval of
 // columns (the reference's test is monotone in |dx|).  One wave handles one
 // point: columns of the NARROW box (tiled:2716-2727) are processed in chunks
-// of EC_CW (1024), with a float-float prefix sum of exp2(dx2 a) in LDS; lanes then
+// of EC_CW (1024), with an FP64 prefix sum of exp2(dx2 a) in LDS; lanes then
 // take rows, find each row's exact column interval, intersect it with the
 // row's valid-cell runs and add exp2(dy2 a) * (prefix differences).
 // Output ew = 1/(cellArea * sum / (2 pi h2f)); points whose boundary distance
@@ -1826,8 +1827,7 @@ __device__ __forceinline__ int64_t first_run_ending_at_or_after(const int2 *runs
 template <int EC_CW>
 __global__ __launch_bounds__(64) void k_ec(EcArgs A)
 {
-    __shared__ float s_ph[EC_CW];
-    __shared__ float s_pl[EC_CW];
+    __shared__ double s_pd[EC_CW];   // prefix sums of Ex over the chunk's columns
 
     const int lane = threadIdx.x;
     const int p = A.list[blockIdx.x];
@@ -1841,7 +1841,7 @@ __global__ __launch_bounds__(64) void k_ec(EcArgs A)
 
     int rl, ru, cl, cu;
     narrow_box(g, px, py, hh, rl, ru, cl, cu);
-    ff acc = {0.0f, 0.0f};
+    double acc = 0.0;
     unsigned long long cnt = 0;
     if (rl <= ru && cl <= cu) {
         const int cn = clampi((int64_t)floor(((double)px - xll) / cs), cl, cu);
@@ -1851,32 +1851,28 @@ __global__ __launch_bounds__(64) void k_ec(EcArgs A)
             // lane owns columns [ca + lane*per, ...)
             const int per = (w + WAVE - 1) / WAVE;
             const int my0 = lane * per;
-            ff run = {0.0f, 0.0f};
+            // FP64 prefix (53-bit significand, more than the 48 of a
+            // float-float pair; FP64 adds cost one VALU instruction each)
+            double run = 0.0;
             for (int t = 0; t < per; ++t) {
                 const int k = my0 + t;
                 if (k < w) {
                     const float dx = A.colx[ca + k] - px;
                     const float ex = fast_exp2((float)(sqd(dx) * a));
-                    run = ff_add_f(run, ex);
-                    s_ph[k] = run.hi;
-                    s_pl[k] = run.lo;
+                    run += (double)ex;
+                    s_pd[k] = run;
                 }
             }
             // exclusive wave scan of the lane totals
-            ff tot = run;
-            ff inc = tot;
+            double inc = run;
             for (int d = 1; d < WAVE; d <<= 1) {
-                ff o = ff_shfl_up(inc, d);
-                if (lane >= d) inc = ff_add(inc, o);
+                const double o = __shfl_up(inc, d, WAVE);
+                if (lane >= d) inc += o;
             }
-            ff offv = ff_add(inc, ff_neg(tot));
+            const double offv = inc - run;
             for (int t = 0; t < per; ++t) {
                 const int k = my0 + t;
-                if (k < w) {
-                    ff v = ff_add(ff{s_ph[k], s_pl[k]}, offv);
-                    s_ph[k] = v.hi;
-                    s_pl[k] = v.lo;
-                }
+                if (k < w) s_pd[k] += offv;
             }
             __syncthreads();
             // rows
@@ -1891,30 +1887,28 @@ __global__ __launch_bounds__(64) void k_ec(EcArgs A)
                 if (lo < ca) lo = ca;
                 if (hi > cb) hi = cb;
                 if (lo > hi) continue;
-                ff seg = {0.0f, 0.0f};
+                double seg = 0.0;
                 for (int64_t k = first_run_ending_at_or_after(A.runs.run_se, rs, re, lo); k < re; ++k) {
                     const int2 rr = A.runs.run_se[k];
                     if (rr.x > hi) break;
                     const int s = rr.x > lo ? rr.x : lo;
                     const int e = rr.y < hi ? rr.y : hi;
-                    ff ps = ff{s_ph[e - ca], s_pl[e - ca]};
-                    if (s > ca) ps = ff_add(ps, ff_neg(ff{s_ph[s - 1 - ca], s_pl[s - 1 - ca]}));
-                    seg = ff_add(seg, ps);
+                    seg += s_pd[e - ca] - (s > ca ? s_pd[s - 1 - ca] : 0.0);
                     cnt += (unsigned long long)(e - s + 1);
                 }
                 const float ey = fast_exp2((float)(dy2 * a));
-                acc = ff_add(acc, ff_mul_f(seg, ey));
+                acc = __builtin_fma(seg, (double)ey, acc);
             }
             __syncthreads();
         }
     }
     // wave reduction
     for (int d = 32; d >= 1; d >>= 1) {
-        acc = ff_add(acc, ff_shfl_xor(acc, d));
+        acc += __shfl_xor(acc, d, WAVE);
         cnt += __shfl_xor(cnt, d, WAVE);
     }
     if (lane == 0) {
-        const double ec = ((double)acc.hi + (double)acc.lo) * (double)(float)(g.cs * g.cs) /
+        const double ec = acc * (double)(float)(g.cs * g.cs) /
                           (h2 * (double)MKDE_TWO_PI_F);
         A.ew[p] = (float)(1.0 / ec);
         if (A.n_eval) atomicAdd(A.n_eval, cnt);
