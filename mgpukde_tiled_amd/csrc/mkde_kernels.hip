@@ -322,15 +322,16 @@ MKDE_PART4(12, 13, 14, 15)
 //      (FULL), or with Ex replaced by 0 on the lanes with |dx| >= reach[p][row]
 //      (PARTIAL: the exact cut-off fl64(dx2 + dy2) < thr), and skips EMPTY
 //      sub-blocks;
-//   3. the FP32 partial sums are Kahan-folded after every DT_FOLD sub-batches.
+//   3. the FP32 partial sums are added to FP64 cell sums after every DT_FOLD
+//      sub-batches.
 // The order of points is fixed (bins in row-major order, points in bin order)
 // so results are deterministic.
 
 #define DT_SB 32
 #define DT_EYP 68   // floats per point row of the Ey and reach tables (64 + pad)
 #define DT_Q (DT_SB + 256)
-// Sub-batches summed in FP32 before a Kahan fold: at most 64 positive terms,
-// a relative error below 64 * 2^-24 = 3.8e-6 per fold
+// Sub-batches summed in FP32 before the fold into FP64: at most 64 positive
+// terms, a relative error below 64 * 2^-24 = 3.8e-6 per fold
 #define DT_FOLD 2
 
 // stats[0] candidate points classified, [1] FULL blocks, [2] PARTIAL blocks,
@@ -488,9 +489,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         s_wyb[tid - 128] = (tid & 1) ? cell_y(yll, cs, nrm05, wr1) : cell_y(yll, cs, nrm05, wr0);
     }
 
-    float acc[16], sum[16], comp[16];
+    // FP32 partial sums (at most DT_FOLD sub-batches of terms) folded into
+    // FP64 cell sums
+    float acc[16];
+    double sum[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) { acc[j] = 0.0f; sum[j] = 0.0f; comp[j] = 0.0f; }
+    for (int j = 0; j < 16; ++j) { acc[j] = 0.0f; sum[j] = 0.0; }
     unsigned st_cand = 0, st_full = 0, st_part = 0, st_wf = 0, st_wp = 0, st_we = 0;
 
     const PointIndex &I = A.idx;
@@ -504,7 +508,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     auto fold = [&]() {
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-            kahan_fold(acc[j], sum[j], comp[j]);
+            sum[j] += (double)acc[j];
             acc[j] = 0.0f;
         }
         npend = 0;
@@ -715,18 +719,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     if (npend > 0) fold();
 
     // write back: nodata cells keep the mask value; parts of a split block
-    // leave FP64 partials (sum - compensation) for k_density_merge
+    // leave FP64 partial sums for k_density_merge
     if (item.z > 1) {
         double *pp = A.partials + (int64_t)item.w * 4096;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) pp[(wave * 16 + j) * 64 + lane] = (double)sum[j] - (double)comp[j];
+        for (int j = 0; j < 16; ++j) pp[(wave * 16 + j) * 64 + lane] = sum[j];
     } else if (col <= c1) {
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             const int64_t row = r0 + wave * 16 + j;
             if (row <= r1) {
                 const int64_t ix = row * g.ncols + col;
-                A.out[ix] = ((vbits >> j) & 1u) ? sum[j] : A.mask[ix];
+                A.out[ix] = ((vbits >> j) & 1u) ? (float)sum[j] : A.mask[ix];
             }
         }
     }
