@@ -50,10 +50,14 @@ hipError_t dist_boundary(hipStream_t, int64_t, const float *, const float *, con
 hipError_t bt_build(hipStream_t, const GridDesc &, const uint8_t *, uint4 *, float4 *, const BndTree &);
 hipError_t dist_tree(hipStream_t, int64_t, const float *, const float *, const int *, const GridDesc &,
                      const BndTree &, double *);
-hipError_t ec_active(hipStream_t, int64_t, const double *, const float *, float, float *, int *, int *);
+hipError_t ec_active(hipStream_t, int64_t, const GridDesc &, const float *, const float *, const double *,
+                     const float *, float, float *, int *, int *, int, int);
+int ec_chunk();
+int ec_wide_min();
+int ec_wide_max();
 hipError_t centres(hipStream_t, const GridDesc &, float *, float *);
-hipError_t ec(hipStream_t, int, const GridDesc &, const MaskRuns &, const float *, const float *, const float *,
-              const float *, const float *, float, const int *, float *, unsigned long long *);
+hipError_t ec(hipStream_t, int64_t, int, int, int, const GridDesc &, const MaskRuns &, const float *, const float *,
+              const float *, const float *, const float *, float, const int *, float *, unsigned long long *);
 hipError_t count_pairs(hipStream_t, const GridDesc &, const MaskRuns &, const PointIndex &, unsigned long long *);
 hipError_t reach(hipStream_t, int, const double *, double *);
 hipError_t gml(hipStream_t, const double *, int64_t, double *);
@@ -748,17 +752,18 @@ static int run_ec(mkde_ctx *c, const float *h, float hf, float *ew_out, unsigned
 {
     CHK(ensure_dist(c));
     CHK(c->list.ensure(c->n));
-    CHK(c->icount.ensure(1));
+    CHK(c->icount.ensure(3));
     CHK(c->ucount.ensure(1));
-    HIPCHK(hipMemsetAsync(c->icount.p, 0, sizeof(int), c->stream));
+    HIPCHK(hipMemsetAsync(c->icount.p, 0, 3 * sizeof(int), c->stream));
     HIPCHK(hipMemsetAsync(c->ucount.p, 0, sizeof(unsigned long long), c->stream));
-    HIPCHK(mkde_dev::ec_active(c->stream, c->n, c->dist.p, h, hf, ew_out, c->list.p, c->icount.p));
-    int nact = 0;
-    CHK(copy_d2h(c, &nact, c->icount.p, sizeof nact));
+    HIPCHK(mkde_dev::ec_active(c->stream, c->n, c->mg, c->x.p, c->y.p, c->dist.p, h, hf, ew_out, c->list.p,
+                               c->icount.p, mkde_dev::ec_wide_min(), mkde_dev::ec_wide_max()));
+    int cnt3[3] = {0, 0, 0};   // narrow points, wide points, widest box
+    CHK(copy_d2h(c, cnt3, c->icount.p, sizeof cnt3));
     MaskRuns R{c->run_ptr.p, c->run_se.p};
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(mkde_dev::ec(c->stream, nact, c->mg, R, c->colx.p, c->rowy.p, c->x.p, c->y.p, h, hf, c->list.p, ew_out,
-                        c->ucount.p));
+    HIPCHK(mkde_dev::ec(c->stream, c->n, cnt3[0], cnt3[1], cnt3[2], c->mg, R, c->colx.p, c->rowy.p, c->x.p, c->y.p,
+                        h, hf, c->list.p, ew_out, c->ucount.p));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     if (n_eval) CHK(copy_d2h(c, n_eval, c->ucount.p, sizeof *n_eval));
     HIPCHK(hipEventSynchronize(c->ev1));

@@ -1716,21 +1716,6 @@ struct EcArgs {
     unsigned long long *n_eval;
 };
 
-__global__ void k_ec_active(int64_t n, const double *dist, const float *h, float h_fixed, float *ew, int *list,
-                            int *count)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float hh = h_fixed > 0.0f ? h_fixed : h[i];
-    const double h2 = (double)(float)(hh * hh);
-    if (dist[i] >= 9.0 * h2) {
-        ew[i] = 1.0f;
-    } else {
-        int k = atomicAdd(count, 1);
-        list[k] = (int)i;
-    }
-}
-
 // NARROW box with the reference's macro semantics (Utilities.h:16-17,30-31;
 // tiled:2716-2727): size_t arithmetic for row_upper / col_upper.
 __device__ __forceinline__ void narrow_box(const GridDesc &g, float px, float py, float h, int &rl, int &ru, int &cl,
@@ -1808,6 +1793,35 @@ __device__ __forceinline__ void row_interval(const COL &colx, double xll, double
     hi = h_e;
 }
 
+// Points whose boundary distance is below the cut-off get a work-list entry:
+// count[0] narrow points from the front of list (box at most cw_narrow
+// columns wide: k_ec, one wave per point, column chunks of cw_narrow),
+// count[1] wide points from the back (wider boxes up to cw_wide columns:
+// k_ec_wide, the whole box in LDS), count[2] the widest wide box.
+__global__ void k_ec_active(int64_t n, GridDesc g, const float *x, const float *y, const double *dist, const float *h,
+                            float h_fixed, float *ew, int *list, int *count, int cw_narrow, int cw_wide)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float hh = h_fixed > 0.0f ? h_fixed : h[i];
+    const double h2 = (double)(float)(hh * hh);
+    if (dist[i] >= 9.0 * h2) {
+        ew[i] = 1.0f;
+        return;
+    }
+    int rl, ru, cl, cu;
+    narrow_box(g, x[i], y[i], hh, rl, ru, cl, cu);
+    const int w = cu - cl + 1;
+    if (rl <= ru && w > cw_narrow && w <= cw_wide) {
+        const int k = atomicAdd(&count[1], 1);
+        list[n - 1 - k] = (int)i;
+        atomicMax(&count[2], w);
+    } else {
+        const int k = atomicAdd(&count[0], 1);
+        list[k] = (int)i;
+    }
+}
+
 // Cell-centre tables of a grid: colx[c] = cell_x(c), rowy[r] = cell_y(r).
 __global__ void k_centres(GridDesc g, float *colx, float *rowy)
 {
@@ -1826,6 +1840,37 @@ __device__ __forceinline__ int64_t first_run_ending_at_or_after(const int2 *runs
         else hi = mid;
     }
     return lo;
+}
+
+// One row r of the edge-correction integral of a point: the exact column
+// interval of the row (row_interval) clipped to the columns [ca, cb] whose
+// prefix sums of Ex are in s_pd, intersected with the row's valid runs;
+// acc += exp2(dy2 a) * (sum of Ex over those cells), cnt += their number.
+__device__ __forceinline__ void ec_row(const EcArgs &A, const double *s_pd, int ca, int cb, int cl, int cu, int cn,
+                                       int r, float px, float py, double thr, double a, double xll, double inv_cs,
+                                       double &acc, unsigned long long &cnt)
+{
+    const float cy = A.rowy[r];
+    const double dy2 = sqd(cy - py);
+    if (!(dy2 < thr)) return;
+    // the row's run range is loaded alongside the interval search
+    const int64_t rs = A.runs.run_ptr[r], re = A.runs.run_ptr[r + 1];
+    int lo, hi;
+    row_interval(ColTable{A.colx}, xll, inv_cs, cl, cu, cn, px, dy2, thr, lo, hi);
+    if (lo < ca) lo = ca;
+    if (hi > cb) hi = cb;
+    if (lo > hi) return;
+    double seg = 0.0;
+    for (int64_t k = first_run_ending_at_or_after(A.runs.run_se, rs, re, lo); k < re; ++k) {
+        const int2 rr = A.runs.run_se[k];
+        if (rr.x > hi) break;
+        const int s = rr.x > lo ? rr.x : lo;
+        const int e = rr.y < hi ? rr.y : hi;
+        seg += s_pd[e - ca] - (s > ca ? s_pd[s - 1 - ca] : 0.0);
+        cnt += (unsigned long long)(e - s + 1);
+    }
+    const float ey = fast_exp2((float)(dy2 * a));
+    acc = __builtin_fma(seg, (double)ey, acc);
 }
 
 template <int EC_CW>
@@ -1880,29 +1925,8 @@ __global__ __launch_bounds__(64) void k_ec(EcArgs A)
             }
             __syncthreads();
             // rows
-            for (int r = rl + lane; r <= ru; r += WAVE) {
-                const float cy = A.rowy[r];
-                const double dy2 = sqd(cy - py);
-                if (!(dy2 < thr)) continue;
-                // the row's run range is loaded alongside the interval search
-                const int64_t rs = A.runs.run_ptr[r], re = A.runs.run_ptr[r + 1];
-                int lo, hi;
-                row_interval(ColTable{A.colx}, xll, inv_cs, cl, cu, cn, px, dy2, thr, lo, hi);
-                if (lo < ca) lo = ca;
-                if (hi > cb) hi = cb;
-                if (lo > hi) continue;
-                double seg = 0.0;
-                for (int64_t k = first_run_ending_at_or_after(A.runs.run_se, rs, re, lo); k < re; ++k) {
-                    const int2 rr = A.runs.run_se[k];
-                    if (rr.x > hi) break;
-                    const int s = rr.x > lo ? rr.x : lo;
-                    const int e = rr.y < hi ? rr.y : hi;
-                    seg += s_pd[e - ca] - (s > ca ? s_pd[s - 1 - ca] : 0.0);
-                    cnt += (unsigned long long)(e - s + 1);
-                }
-                const float ey = fast_exp2((float)(dy2 * a));
-                acc = __builtin_fma(seg, (double)ey, acc);
-            }
+            for (int r = rl + lane; r <= ru; r += WAVE)
+                ec_row(A, s_pd, ca, cb, cl, cu, cn, r, px, py, thr, a, xll, inv_cs, acc, cnt);
             __syncthreads();
         }
     }
@@ -1916,6 +1940,80 @@ __global__ __launch_bounds__(64) void k_ec(EcArgs A)
                           (h2 * (double)MKDE_TWO_PI_F);
         A.ew[p] = (float)(1.0 / ec);
         if (A.n_eval) atomicAdd(A.n_eval, cnt);
+    }
+}
+
+// Wide points (NARROW box wider than the column chunk of k_ec): one
+// 256-thread workgroup per point and the whole box's prefix sums in dynamic
+// LDS (8 B per column), so every row is visited once instead of once per
+// chunk.  Same row arithmetic as k_ec; the prefix is scanned across the
+// workgroup (four waves).
+__global__ __launch_bounds__(256) void k_ec_wide(EcArgs A, int64_t n)
+{
+    extern __shared__ double s_pd[];
+    __shared__ double s_wt[4];
+    __shared__ double s_acc[4];
+    __shared__ unsigned long long s_cnt[4];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int p = A.list[n - 1 - blockIdx.x];
+    const GridDesc g = A.g;
+    const double xll = g.xll, cs = g.cs, inv_cs = 1.0 / cs;
+    const float px = A.x[p], py = A.y[p];
+    const float hh = A.h_fixed > 0.0f ? A.h_fixed : A.h[p];
+    const double h2 = (double)(float)(hh * hh);
+    const double thr = 9.0 * h2;
+    const double a = -MKDE_LOG2E / (2.0 * h2);
+
+    int rl, ru, cl, cu;
+    narrow_box(g, px, py, hh, rl, ru, cl, cu);   // non-empty: k_ec_active checked
+    const int cn = clampi((int64_t)floor(((double)px - xll) / cs), cl, cu);
+    const int w = cu - cl + 1;
+    const int per = (w + 255) / 256;
+    const int my0 = tid * per;
+    double run = 0.0;
+    for (int t = 0; t < per; ++t) {
+        const int k = my0 + t;
+        if (k < w) {
+            const float dx = A.colx[cl + k] - px;
+            const float ex = fast_exp2((float)(sqd(dx) * a));
+            run += (double)ex;
+            s_pd[k] = run;
+        }
+    }
+    // exclusive scan of the thread totals: within each wave, then the waves
+    double inc = run;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const double o = __shfl_up(inc, d, WAVE);
+        if (lane >= d) inc += o;
+    }
+    if (lane == WAVE - 1) s_wt[wave] = inc;
+    __syncthreads();
+    double woff = 0.0;
+    for (int v = 0; v < wave; ++v) woff += s_wt[v];
+    const double offv = woff + (inc - run);
+    for (int t = 0; t < per; ++t) {
+        const int k = my0 + t;
+        if (k < w) s_pd[k] += offv;
+    }
+    __syncthreads();
+    double acc = 0.0;
+    unsigned long long cnt = 0;
+    for (int r = rl + tid; r <= ru; r += 256) ec_row(A, s_pd, cl, cu, cl, cu, cn, r, px, py, thr, a, xll, inv_cs, acc, cnt);
+    for (int d = 32; d >= 1; d >>= 1) {
+        acc += __shfl_xor(acc, d, WAVE);
+        cnt += __shfl_xor(cnt, d, WAVE);
+    }
+    if (lane == 0) {
+        s_acc[wave] = acc;
+        s_cnt[wave] = cnt;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const double tot = ((s_acc[0] + s_acc[1]) + s_acc[2]) + s_acc[3];
+        const double ec = tot * (double)(float)(g.cs * g.cs) / (h2 * (double)MKDE_TWO_PI_F);
+        A.ew[p] = (float)(1.0 / ec);
+        if (A.n_eval) atomicAdd(A.n_eval, s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3]);
     }
 }
 
@@ -2275,11 +2373,12 @@ hipError_t dist_boundary(hipStream_t s, int64_t n, const float *x, const float *
     return hipGetLastError();
 }
 
-hipError_t ec_active(hipStream_t s, int64_t n, const double *dist, const float *h, float h_fixed, float *ew,
-                     int *list, int *count)
+hipError_t ec_active(hipStream_t s, int64_t n, const GridDesc &g, const float *x, const float *y, const double *dist,
+                     const float *h, float h_fixed, float *ew, int *list, int *count, int cw_narrow, int cw_wide)
 {
-    if (n > 0) hipLaunchKernelGGL(k_ec_active, dim3(nblk(n, 256)), dim3(256), 0, s, n, dist, h, h_fixed, ew, list,
-                                  count);
+    if (n > 0)
+        hipLaunchKernelGGL(k_ec_active, dim3(nblk(n, 256)), dim3(256), 0, s, n, g, x, y, dist, h, h_fixed, ew, list,
+                           count, cw_narrow, cw_wide);
     return hipGetLastError();
 }
 
@@ -2290,22 +2389,51 @@ hipError_t centres(hipStream_t s, const GridDesc &g, float *colx, float *rowy)
     return hipGetLastError();
 }
 
-hipError_t ec(hipStream_t s, int nactive, const GridDesc &g, const MaskRuns &runs, const float *colx,
-              const float *rowy, const float *x, const float *y, const float *h, float h_fixed, const int *list,
-              float *ew, unsigned long long *n_eval)
+// Column chunk of k_ec's prefix sums (LDS 8 B per column and wave); 1024
+// measured best (512 / 1024 / 2048: the adaptive job's EC within 1%, the
+// 64-candidate search 314 / 296 / 307 ms); MKDE_EC_CHUNK overrides it.
+int ec_chunk()
 {
-    EcArgs A{g, runs, colx, rowy, x, y, h, h_fixed, list, ew, n_eval};
-    // column chunk of the prefix sums (LDS 8 B per column and wave); 1024
-    // measured best (512 / 1024 / 2048: the adaptive job's EC within 1%, the
-    // 64-candidate search 314 / 296 / 307 ms); MKDE_EC_CHUNK overrides it
     static const int cw = [] {
         const char *e = getenv("MKDE_EC_CHUNK");
-        return e ? atoi(e) : 1024;
+        const int v = e ? atoi(e) : 1024;
+        return (v == 512 || v == 2048) ? v : 1024;
     }();
-    if (nactive > 0) {
-        if (cw == 512) hipLaunchKernelGGL(k_ec<512>, dim3((unsigned)nactive), dim3(64), 0, s, A);
-        else if (cw == 1024) hipLaunchKernelGGL(k_ec<1024>, dim3((unsigned)nactive), dim3(64), 0, s, A);
-        else hipLaunchKernelGGL(k_ec<2048>, dim3((unsigned)nactive), dim3(64), 0, s, A);
+    return cw;
+}
+
+// Widest box k_ec_wide takes: its prefix sums fill at most 64,000 B of LDS.
+int ec_wide_max() { return 8000; }
+
+// Boxes wider than this go to k_ec_wide (default: the chunk width, i.e. every
+// box k_ec would cut into two or more chunks); MKDE_EC_WIDE_MIN overrides it.
+int ec_wide_min()
+{
+    static const int v = [] {
+        const char *e = getenv("MKDE_EC_WIDE_MIN");
+        const int x = e ? atoi(e) : 0;
+        return x > 0 ? x : 0;
+    }();
+    return v > 0 ? v : ec_chunk();
+}
+
+// The n_narrow points at the front of list run in k_ec (one wave each, column
+// chunks), the n_wide points at the back in k_ec_wide (one workgroup each,
+// max_w columns of LDS).
+hipError_t ec(hipStream_t s, int64_t n, int n_narrow, int n_wide, int max_w, const GridDesc &g, const MaskRuns &runs,
+              const float *colx, const float *rowy, const float *x, const float *y, const float *h, float h_fixed,
+              const int *list, float *ew, unsigned long long *n_eval)
+{
+    EcArgs A{g, runs, colx, rowy, x, y, h, h_fixed, list, ew, n_eval};
+    const int cw = ec_chunk();
+    if (n_narrow > 0) {
+        if (cw == 512) hipLaunchKernelGGL(k_ec<512>, dim3((unsigned)n_narrow), dim3(64), 0, s, A);
+        else if (cw == 1024) hipLaunchKernelGGL(k_ec<1024>, dim3((unsigned)n_narrow), dim3(64), 0, s, A);
+        else hipLaunchKernelGGL(k_ec<2048>, dim3((unsigned)n_narrow), dim3(64), 0, s, A);
+    }
+    if (n_wide > 0) {
+        if (max_w < 1 || max_w > ec_wide_max()) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_ec_wide, dim3((unsigned)n_wide), dim3(256), (size_t)max_w * sizeof(double), s, A, n);
     }
     return hipGetLastError();
 }
