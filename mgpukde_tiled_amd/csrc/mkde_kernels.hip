@@ -2271,7 +2271,15 @@ hipError_t point_plan(hipStream_t s, const PointIndex &idx, int kind, float h_fi
     const unsigned nred = (unsigned)std::min<int64_t>(1024, ((int64_t)nw + 255) / 256);
     hipLaunchKernelGGL(k_cost_total, dim3(nred), dim3(256), 0, s, nw, W.cost, W.total);
     const unsigned tmin = 4096;
-    const unsigned long long target = kind == 1 ? 2048ull : 16384ull;
+    // split target of the 16-candidate leave-one-out pass: 32768 measured best
+    // on the 200k-point 64-candidate search (2048 / 8192 / 32768 / 65536 /
+    // 131072: 237 / 200 / 187 / 191 / 194 ms); MKDE_LOO_TARGET overrides it
+    static const unsigned long long target_loo = [] {
+        const char *e = getenv("MKDE_LOO_TARGET");
+        const long long v = e ? atoll(e) : 0;
+        return v > 0 ? (unsigned long long)v : 32768ull;
+    }();
+    const unsigned long long target = kind == 1 ? target_loo : 16384ull;
     hipLaunchKernelGGL(k_parts, dim3(nblk_(nw)), dim3(256), 0, s, nw, W.cost, W.total, tmin, target, W.parts,
                        W.item0, W.sidx);
     scan_u32(s, nw, W.item0, W.slot0, W.chunks, W.counts + 1);

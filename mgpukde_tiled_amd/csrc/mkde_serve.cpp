@@ -117,12 +117,13 @@ extern "C" int64_t mkde_job_footprint(int64_t n, int64_t ncols, int64_t nrows, i
     // mask, valid flags, boundary flags and list, row runs, density output
     b += 24 * ncols * nrows + 16 * nrows;
     // FP64 partials of split 64-point waves of the point kernels: at most
-    // 2 * cost / 4096 parts per wave (cost <= n) and at most 2 * 16384 (density
-    // at points, 512 B each) or 2 * 2048 (16-candidate pass, 8 KB each)
+    // 2 * cost / 4096 parts per wave (cost <= n) and at most twice the split
+    // target: 2 * 16384 (density at points, 512 B each) or 2 * 32768
+    // (16-candidate pass, 8 KB each; MKDE_LOO_TARGET is not taken into account)
     const int64_t nw = (n + 63) / 64;
     const double parts = 2.0 * (double)nw * (double)n / 4096.0;
     b += (int64_t)std::min(parts, 32768.0) * 512;
-    if (loo) b += (int64_t)std::min(parts, 4096.0) * 8192;
+    if (loo) b += (int64_t)std::min(parts, 65536.0) * 8192;
     return b;
 }
 

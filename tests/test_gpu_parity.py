@@ -297,3 +297,32 @@ def test_density_at_points_cutoff_ties(ctx, orc, adaptive):
     pos = o["den1_64"] > 0
     assert np.all(d1[~pos] == 0)
     assert rel_err(d1[pos], o["den1_64"][pos]).max() <= REL
+
+
+@pytest.mark.parametrize("hs_range", [(180.0, 180.0), (450.0, 450.0), (40.0, 450.0)])
+def test_ec_weights_wide_boxes(ctx, orc, hs_range):
+    # boxes wider than one 1024-column chunk go to the wide EC kernel (whole
+    # box in LDS); an adaptive mix sends points to both kernels in one pass
+    import mgpukde_tiled_amd as M
+
+    nc, nr = 1400, 900
+    g = orc.grid(nc, nr, 100.0, 50.0, 1.0)
+    r = np.random.default_rng(17)
+    mask = np.ones((nr, nc), np.float32)
+    mask[r.random((nr, nc)) < 0.01] = -9999.0
+    mask[200:420, 300:700] = -9999.0
+    mask[:, 1300:] = -9999.0
+    n = 160
+    x = (100.0 + r.uniform(0, nc, n)).astype(np.float32)
+    y = (50.0 + r.uniform(0, nr, n)).astype(np.float32)
+    w = r.uniform(0.5, 2.0, n).astype(np.float32)
+    hs = r.uniform(hs_range[0], hs_range[1], n).astype(np.float32)
+    ctx.points_upload(x, y, w)
+    ctx.mask_upload(mgrid(M, g), mask)
+    ctx.set_bandwidths(hs)
+    ew, ne = ctx.EdgeCorrectionWeightsExact()
+    oflags, _ = orc.mark_boundary(g, mask)
+    od = orc.dist_to_boundary(x, y, g, oflags)
+    _, ew64, one = orc.ec_weights(x, y, od, hs, g, mask)
+    assert ne == one
+    assert rel_err(ew, ew64).max() <= REL
