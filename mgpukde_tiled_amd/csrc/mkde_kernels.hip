@@ -2206,7 +2206,15 @@ hipError_t density_plan(hipStream_t s, const GridDesc &g, const float *mask, con
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, (int64_t)2, 0.0f, (float *)W.total);
     const unsigned nred = (unsigned)std::min<int64_t>(1024, ((int64_t)nblk + 255) / 256);
     hipLaunchKernelGGL(k_cost_total, dim3(nred), dim3(256), 0, s, nblk, W.cost, W.total);
-    hipLaunchKernelGGL(k_parts, dim3(nblk_(nblk)), dim3(256), 0, s, nblk, W.cost, W.total, DT_TMIN, DT_TARGET, W.parts,
+    // measured (adaptive 1M job / 10M-point tiles, density ms): target 1024:
+    // 30.9 / 32.7, 2048: 30.9 / 27.5, 4096: 31.0 / 27.8, 16384: 31.7 / 27.9,
+    // 65536: 33.3 / 29.2; MKDE_DENSITY_TARGET overrides the default 4096
+    static const unsigned long long dt_target = [] {
+        const char *e = getenv("MKDE_DENSITY_TARGET");
+        const long long v = e ? atoll(e) : 0;
+        return v > 0 ? (unsigned long long)v : DT_TARGET;
+    }();
+    hipLaunchKernelGGL(k_parts, dim3(nblk_(nblk)), dim3(256), 0, s, nblk, W.cost, W.total, DT_TMIN, dt_target, W.parts,
                        W.item0, W.sidx);
     // in place is not allowed: parts -> item0 scan reads parts; sparts / sflag
     // were written to item0 / sidx and are scanned into slot0 / sidx2 below
