@@ -2287,7 +2287,15 @@ hipError_t point_plan(hipStream_t s, const PointIndex &idx, int kind, float h_fi
         const long long v = e ? atoll(e) : 0;
         return v > 0 ? (unsigned long long)v : 32768ull;
     }();
-    const unsigned long long target = kind == 1 ? target_loo : 16384ull;
+    // point densities (pilot): 16384 measured best on the adaptive 1M job
+    // (4096 / 16384 / 65536: bandwidth phase 8.2 / 7.5 / 8.0 ms);
+    // MKDE_PTS_TARGET overrides it
+    static const unsigned long long target_pts = [] {
+        const char *e = getenv("MKDE_PTS_TARGET");
+        const long long v = e ? atoll(e) : 0;
+        return v > 0 ? (unsigned long long)v : 16384ull;
+    }();
+    const unsigned long long target = kind == 1 ? target_loo : target_pts;
     hipLaunchKernelGGL(k_parts, dim3(nblk_(nw)), dim3(256), 0, s, nw, W.cost, W.total, tmin, target, W.parts,
                        W.item0, W.sidx);
     scan_u32(s, nw, W.item0, W.slot0, W.chunks, W.counts + 1);
