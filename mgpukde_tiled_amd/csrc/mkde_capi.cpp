@@ -59,6 +59,8 @@ hipError_t centres(hipStream_t, const GridDesc &, float *, float *);
 hipError_t ec(hipStream_t, int64_t, int, int, int, const GridDesc &, const MaskRuns &, const float *, const float *,
               const float *, const float *, const float *, float, const int *, float *, unsigned long long *);
 hipError_t count_pairs(hipStream_t, const GridDesc &, const MaskRuns &, const PointIndex &, unsigned long long *);
+int64_t slot_words();
+hipError_t fold_slots(hipStream_t, unsigned long long *);
 hipError_t reach(hipStream_t, int, const double *, double *);
 hipError_t gml(hipStream_t, const double *, int64_t, double *);
 hipError_t sumlog(hipStream_t, int64_t, int, const float *, int, double *, int, double *);
@@ -753,9 +755,10 @@ static int run_ec(mkde_ctx *c, const float *h, float hf, float *ew_out, unsigned
     CHK(ensure_dist(c));
     CHK(c->list.ensure(c->n));
     CHK(c->icount.ensure(3));
-    CHK(c->ucount.ensure(1));
+    const int64_t nw = mkde_dev::slot_words();
+    CHK(c->ucount.ensure(nw));
     HIPCHK(hipMemsetAsync(c->icount.p, 0, 3 * sizeof(int), c->stream));
-    HIPCHK(hipMemsetAsync(c->ucount.p, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(hipMemsetAsync(c->ucount.p, 0, nw * sizeof(unsigned long long), c->stream));
     HIPCHK(mkde_dev::ec_active(c->stream, c->n, c->mg, c->x.p, c->y.p, c->dist.p, h, hf, ew_out, c->list.p,
                                c->icount.p, mkde_dev::ec_wide_min(), mkde_dev::ec_wide_max()));
     int cnt3[3] = {0, 0, 0};   // narrow points, wide points, widest box
@@ -765,7 +768,10 @@ static int run_ec(mkde_ctx *c, const float *h, float hf, float *ew_out, unsigned
     HIPCHK(mkde_dev::ec(c->stream, c->n, cnt3[0], cnt3[1], cnt3[2], c->mg, R, c->colx.p, c->rowy.p, c->x.p, c->y.p,
                         h, hf, c->list.p, ew_out, c->ucount.p));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
-    if (n_eval) CHK(copy_d2h(c, n_eval, c->ucount.p, sizeof *n_eval));
+    if (n_eval) {
+        HIPCHK(mkde_dev::fold_slots(c->stream, c->ucount.p));
+        CHK(copy_d2h(c, n_eval, c->ucount.p + nw - 1, sizeof *n_eval));
+    }
     HIPCHK(hipEventSynchronize(c->ev1));
     HIPCHK(hipEventElapsedTime(&c->last_ec_ms, c->ev0, c->ev1));
     return MKDE_OK;
@@ -1318,14 +1324,16 @@ extern "C" int mkde_count_density_pairs(mkde_ctx *c, const mkde_grid *tile, cons
     CHK(copy_h2d(c, rp.p, R.ptr.data(), R.ptr.size() * sizeof(int64_t)));
     CHK(copy_h2d(c, rs.p, R.se.data(), R.se.size() * sizeof(int2)));
     CHK(prep(c));
-    CHK(c->ucount.ensure(1));
-    HIPCHK(hipMemsetAsync(c->ucount.p, 0, sizeof(unsigned long long), c->stream));
+    const int64_t nw = mkde_dev::slot_words();
+    CHK(c->ucount.ensure(nw));
+    HIPCHK(hipMemsetAsync(c->ucount.p, 0, nw * sizeof(unsigned long long), c->stream));
     MaskRuns MR{rp.p, rs.p};
     PointIndex I = index_of(c);
     GridDesc g = to_desc(tile);
     HIPCHK(mkde_dev::count_pairs(c->stream, g, MR, I, c->ucount.p));
+    HIPCHK(mkde_dev::fold_slots(c->stream, c->ucount.p));
     unsigned long long t = 0;
-    CHK(copy_d2h(c, &t, c->ucount.p, sizeof t));
+    CHK(copy_d2h(c, &t, c->ucount.p + nw - 1, sizeof t));
     *n_pairs = (int64_t)t;
     return MKDE_OK;
 }

@@ -83,6 +83,17 @@ __device__ __forceinline__ float far1(float v, float lo, float hi)
     return a > b ? a : b;
 }
 
+// Launch-wide counters spread over MKDE_CNT_SLOTS cache lines: atomics of
+// many workgroups to one address serialise (measured: 400k one-per-workgroup
+// adds to a single counter cost ~2.6 ms), so workgroup b adds to slot
+// b % MKDE_CNT_SLOTS and k_fold_slots sums the slots into the word after them.
+#define MKDE_CNT_SLOTS 512
+#define MKDE_CNT_STRIDE 16   // 64-bit words between slots (128 B)
+__device__ __forceinline__ void slot_add(unsigned long long *slots, unsigned long long v)
+{
+    if (v) atomicAdd(&slots[(blockIdx.x % MKDE_CNT_SLOTS) * MKDE_CNT_STRIDE], v);
+}
+
 // Kahan fold of a short FP32 partial sum into (sum, comp).
 __device__ __forceinline__ void kahan_fold(float part, float &sum, float &comp)
 {
@@ -1939,7 +1950,7 @@ __global__ __launch_bounds__(64) void k_ec(EcArgs A)
         const double ec = acc * (double)(float)(g.cs * g.cs) /
                           (h2 * (double)MKDE_TWO_PI_F);
         A.ew[p] = (float)(1.0 / ec);
-        if (A.n_eval) atomicAdd(A.n_eval, cnt);
+        if (A.n_eval) slot_add(A.n_eval, cnt);
     }
 }
 
@@ -2013,7 +2024,7 @@ __global__ __launch_bounds__(256) void k_ec_wide(EcArgs A, int64_t n)
         const double tot = ((s_acc[0] + s_acc[1]) + s_acc[2]) + s_acc[3];
         const double ec = tot * (double)(float)(g.cs * g.cs) / (h2 * (double)MKDE_TWO_PI_F);
         A.ew[p] = (float)(1.0 / ec);
-        if (A.n_eval) atomicAdd(A.n_eval, s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3]);
+        if (A.n_eval) slot_add(A.n_eval, s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3]);
     }
 }
 
@@ -2059,7 +2070,7 @@ __global__ __launch_bounds__(64) void k_count_pairs(CountArgs A)
         }
     }
     for (int d = 32; d >= 1; d >>= 1) cnt += __shfl_xor(cnt, d, WAVE);
-    if (lane == 0 && cnt) atomicAdd(A.total, cnt);
+    if (lane == 0) slot_add(A.total, cnt);
 }
 
 // reach = sqrt(max bin threshold) (world units), slightly enlarged so that the
@@ -2459,6 +2470,27 @@ hipError_t ec(hipStream_t s, int64_t n, int n_narrow, int n_wide, int max_w, con
         if (max_w < 1 || max_w > ec_wide_max()) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_ec_wide, dim3((unsigned)n_wide), dim3(256), (size_t)max_w * sizeof(double), s, A, n);
     }
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_fold_slots(unsigned long long *slots)
+{
+    __shared__ unsigned long long s_w[4];
+    unsigned long long v = 0;
+    for (int k = threadIdx.x; k < MKDE_CNT_SLOTS; k += 256) v += slots[k * MKDE_CNT_STRIDE];
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, WAVE);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) slots[MKDE_CNT_SLOTS * MKDE_CNT_STRIDE] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// Words of a slot counter (slots, then the folded total).
+int64_t slot_words() { return (int64_t)MKDE_CNT_SLOTS * MKDE_CNT_STRIDE + 1; }
+
+// slots[slot_words() - 1] = sum of the slots.
+hipError_t fold_slots(hipStream_t s, unsigned long long *slots)
+{
+    hipLaunchKernelGGL(k_fold_slots, dim3(1), dim3(256), 0, s, slots);
     return hipGetLastError();
 }
 
