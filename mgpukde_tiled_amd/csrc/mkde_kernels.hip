@@ -1504,13 +1504,16 @@ __global__ void k_valid(int64_t ncells, const float *mask, float nodata, uint8_t
 // MarkBoundary: a valid cell is a boundary cell iff it is on the outer rows /
 // columns or one of its 8 neighbours is nodata (tiled:3993-4059).  Boundary
 // cell centres are appended to blist (order irrelevant: only minima are taken).
-__global__ void k_boundary(GridDesc g, const uint8_t *valid, uint8_t *flags, float2 *blist,
-                           unsigned long long *bcount)
+// Row r = blockIdx.y (+ gridDim.y strides), column from blockIdx.x: no
+// 64-bit division per cell.
+__global__ __launch_bounds__(256) void k_boundary(GridDesc g, const uint8_t *valid, uint8_t *flags, float2 *blist,
+                                                  unsigned long long *bcount)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nc = g.ncols, nr = g.nrows;
-    if (i >= nc * nr) return;
-    const int64_t r = i / nc, c = i - r * nc;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    for (int64_t r = blockIdx.y; r < nr; r += gridDim.y) {
+    const int64_t i = r * nc + c;
     uint8_t f = 0;
     if (valid[i]) {
         if (r == 0 || r == nr - 1 || c == 0 || c == nc - 1) {
@@ -1525,6 +1528,7 @@ __global__ void k_boundary(GridDesc g, const uint8_t *valid, uint8_t *flags, flo
         unsigned long long k = atomicAdd(bcount, 1ull);
         const double nrm05 = (double)nr - 0.5;
         blist[k] = make_float2(cell_x(g.xll, g.cs, c), cell_y(g.yll, g.cs, nrm05, r));
+    }
     }
 }
 
@@ -2116,13 +2120,22 @@ __global__ __launch_bounds__(256) void k_sumlog_partial(int64_t n, const float *
     if (threadIdx.x == 0) partial[(int64_t)k * gridDim.x + blockIdx.x] = ((s[0] + s[1]) + s[2]) + s[3];
 }
 
-__global__ void k_sum_final(int nblk, int K, const double *partial, double *out)
+// One workgroup per candidate: thread t adds partials t, t + 256, ... in
+// order, then a fixed-shape tree over the 256 thread sums (deterministic).
+__global__ __launch_bounds__(256) void k_sum_final(int nblk, int K, const double *partial, double *out)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
+    __shared__ double s_w[256];
+    const int k = blockIdx.x;
+    const double *pk = partial + (int64_t)k * nblk;
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)k * nblk + b];
-    out[k] = s;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += pk[b];
+    s_w[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) s_w[threadIdx.x] += s_w[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = s_w[0];
 }
 
 // Abramson update h_i = h0 * powf(den0_i / g, alpha) (tiled:571, kde_kernel_kdtr.cu:1007-1037),
@@ -2377,8 +2390,9 @@ hipError_t valid_cells(hipStream_t s, int64_t ncells, const float *mask, float n
 hipError_t boundary(hipStream_t s, const GridDesc &g, const uint8_t *valid, uint8_t *flags, float2 *blist,
                     unsigned long long *bcount)
 {
-    int64_t nc = g.ncols * g.nrows;
-    hipLaunchKernelGGL(k_boundary, dim3(nblk(nc, 256)), dim3(256), 0, s, g, valid, flags, blist, bcount);
+    if (g.ncols <= 0 || g.nrows <= 0) return hipSuccess;
+    const unsigned gy = (unsigned)std::min<int64_t>(g.nrows, 65535);
+    hipLaunchKernelGGL(k_boundary, dim3(nblk(g.ncols, 256), gy), dim3(256), 0, s, g, valid, flags, blist, bcount);
     return hipGetLastError();
 }
 
@@ -2517,7 +2531,7 @@ hipError_t gml(hipStream_t s, const double *S, int64_t n, double *g)
 hipError_t sumlog(hipStream_t s, int64_t n, int K, const float *den, int clamp, double *partial, int nb, double *out)
 {
     hipLaunchKernelGGL(k_sumlog_partial, dim3((unsigned)nb, (unsigned)K), dim3(256), 0, s, n, den, clamp, partial);
-    hipLaunchKernelGGL(k_sum_final, dim3(nblk(K, 64)), dim3(64), 0, s, nb, K, partial, out);
+    hipLaunchKernelGGL(k_sum_final, dim3((unsigned)K), dim3(256), 0, s, nb, K, partial, out);
     return hipGetLastError();
 }
 
