@@ -186,7 +186,8 @@ struct mkde_ctx {
     DevBuf<float> colx, rowy;   // cell-centre tables of the mask grid
     DevBuf<uint8_t> bflags;
     DevBuf<float2> blist;
-    DevBuf<unsigned long long> bcount;
+    DevBuf<unsigned long long> bcount;   // counter slots, or one counter with the list
+    bool blist_on = false;               // boundary centre list built (brute-force distances)
     bool bflags_ok = false;    // boundary flags / list enqueued for the current mask
     int64_t nboundary = -1;    // boundary cell count once read back
     // boundary hierarchy (nearest-boundary search)
@@ -610,15 +611,35 @@ static int need_mask(mkde_ctx *c)
 
 // Boundary flags and list are enqueued without a host wait; the count is
 // read back only when a caller needs it (boundary_count).
+// The boundary hierarchy covers rasters up to 16384 leaves (262,144 cells) on
+// each side.
+static bool btree_fits(const mkde_ctx *c)
+{
+    const int64_t nx = (c->mg.ncols + MKDE_BT_S - 1) / MKDE_BT_S, ny = (c->mg.nrows + MKDE_BT_S - 1) / MKDE_BT_S;
+    return nx <= 16384 && ny <= 16384;
+}
+
 static int ensure_boundary(mkde_ctx *c)
 {
     CHK(need_mask(c));
     if (c->bflags_ok) return MKDE_OK;
     const int64_t nc = c->mg.ncols * c->mg.nrows;
-    // upper bound of the boundary list: every valid cell
-    CHK(c->blist.ensure(nc));
-    HIPCHK(hipMemsetAsync(c->bcount.p, 0, sizeof(unsigned long long), c->stream));
-    HIPCHK(mkde_dev::boundary(c->stream, c->mg, c->valid.p, c->bflags.p, c->blist.p, c->bcount.p));
+    // the list of boundary centres only feeds the brute-force distances
+    // (rasters too large for the hierarchy); otherwise only the count is
+    // kept, in counter slots (one atomic per wave to one address serialises
+    // across the device)
+    c->blist_on = !btree_fits(c);
+    const int64_t nw = mkde_dev::slot_words();
+    CHK(c->bcount.ensure(nw));
+    HIPCHK(hipMemsetAsync(c->bcount.p, 0, nw * sizeof(unsigned long long), c->stream));
+    if (c->blist_on) {
+        // upper bound of the boundary list: every valid cell
+        CHK(c->blist.ensure(nc));
+        HIPCHK(mkde_dev::boundary(c->stream, c->mg, c->valid.p, c->bflags.p, c->blist.p, c->bcount.p));
+    } else {
+        HIPCHK(mkde_dev::boundary(c->stream, c->mg, c->valid.p, c->bflags.p, nullptr, c->bcount.p));
+        HIPCHK(mkde_dev::fold_slots(c->stream, c->bcount.p));
+    }
     c->bflags_ok = true;
     c->nboundary = -1;
     return MKDE_OK;
@@ -629,7 +650,7 @@ static int boundary_count(mkde_ctx *c)
     CHK(ensure_boundary(c));
     if (c->nboundary >= 0) return MKDE_OK;
     unsigned long long nb = 0;
-    CHK(copy_d2h(c, &nb, c->bcount.p, sizeof nb));
+    CHK(copy_d2h(c, &nb, c->bcount.p + (c->blist_on ? 0 : mkde_dev::slot_words() - 1), sizeof nb));
     c->nboundary = (int64_t)nb;
     return MKDE_OK;
 }
@@ -652,8 +673,8 @@ static int ensure_btree(mkde_ctx *c)
     if (c->bt_ok) return MKDE_OK;
     BndTree &T = c->bt;
     T = BndTree{};
+    if (!btree_fits(c)) return MKDE_OK;
     int nx = (int)((c->mg.ncols + MKDE_BT_S - 1) / MKDE_BT_S), ny = (int)((c->mg.nrows + MKDE_BT_S - 1) / MKDE_BT_S);
-    if (nx > 16384 || ny > 16384) return MKDE_OK;
     int64_t total = 0;
     for (;;) {
         T.nx[T.nlev] = nx;

@@ -1505,7 +1505,8 @@ __global__ void k_valid(int64_t ncells, const float *mask, float nodata, uint8_t
 // columns or one of its 8 neighbours is nodata (tiled:3993-4059).  Boundary
 // cell centres are appended to blist (order irrelevant: only minima are taken).
 // Row r = blockIdx.y (+ gridDim.y strides), column from blockIdx.x: no
-// 64-bit division per cell.
+// 64-bit division per cell.  With blist the boundary centres are appended
+// (bcount: one counter); without, only counted, in counter slots.
 __global__ __launch_bounds__(256) void k_boundary(GridDesc g, const uint8_t *valid, uint8_t *flags, float2 *blist,
                                                   unsigned long long *bcount)
 {
@@ -1525,9 +1526,13 @@ __global__ __launch_bounds__(256) void k_boundary(GridDesc g, const uint8_t *val
     }
     flags[i] = f;
     if (f) {
-        unsigned long long k = atomicAdd(bcount, 1ull);
-        const double nrm05 = (double)nr - 0.5;
-        blist[k] = make_float2(cell_x(g.xll, g.cs, c), cell_y(g.yll, g.cs, nrm05, r));
+        if (blist) {
+            unsigned long long k = atomicAdd(bcount, 1ull);
+            const double nrm05 = (double)nr - 0.5;
+            blist[k] = make_float2(cell_x(g.xll, g.cs, c), cell_y(g.yll, g.cs, nrm05, r));
+        } else {
+            atomicAdd(&bcount[((blockIdx.y * gridDim.x + blockIdx.x) % MKDE_CNT_SLOTS) * MKDE_CNT_STRIDE], 1ull);
+        }
     }
     }
 }
@@ -1813,28 +1818,41 @@ __device__ __forceinline__ void row_interval(const COL &colx, double xll, double
 // columns wide: k_ec, one wave per point, column chunks of cw_narrow),
 // count[1] wide points from the back (wider boxes up to cw_wide columns:
 // k_ec_wide, the whole box in LDS), count[2] the widest wide box.
-__global__ void k_ec_active(int64_t n, GridDesc g, const float *x, const float *y, const double *dist, const float *h,
-                            float h_fixed, float *ew, int *list, int *count, int cw_narrow, int cw_wide)
+__global__ __launch_bounds__(256) void k_ec_active(int64_t n, GridDesc g, const float *x, const float *y,
+                                                   const double *dist, const float *h, float h_fixed, float *ew,
+                                                   int *list, int *count, int cw_narrow, int cw_wide)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float hh = h_fixed > 0.0f ? h_fixed : h[i];
-    const double h2 = (double)(float)(hh * hh);
-    if (dist[i] >= 9.0 * h2) {
-        ew[i] = 1.0f;
-        return;
+    // list slots are reserved per workgroup (LDS counters, then one global
+    // atomic per list): per-wave atomics on one address serialise
+    __shared__ int s_n[2], s_base[2], s_wmax;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s_n[0] = s_n[1] = 0;
+        s_wmax = 0;
     }
-    int rl, ru, cl, cu;
-    narrow_box(g, x[i], y[i], hh, rl, ru, cl, cu);
-    const int w = cu - cl + 1;
-    if (rl <= ru && w > cw_narrow && w <= cw_wide) {
-        const int k = atomicAdd(&count[1], 1);
-        list[n - 1 - k] = (int)i;
-        atomicMax(&count[2], w);
-    } else {
-        const int k = atomicAdd(&count[0], 1);
-        list[k] = (int)i;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
+    int kind = -1, slot = 0;   // 0 narrow, 1 wide
+    if (i < n) {
+        const float hh = h_fixed > 0.0f ? h_fixed : h[i];
+        const double h2 = (double)(float)(hh * hh);
+        if (dist[i] >= 9.0 * h2) {
+            ew[i] = 1.0f;
+        } else {
+            int rl, ru, cl, cu;
+            narrow_box(g, x[i], y[i], hh, rl, ru, cl, cu);
+            const int w = cu - cl + 1;
+            kind = (rl <= ru && w > cw_narrow && w <= cw_wide) ? 1 : 0;
+            slot = atomicAdd(&s_n[kind], 1);
+            if (kind == 1) atomicMax(&s_wmax, w);
+        }
     }
+    __syncthreads();
+    if (tid < 2 && s_n[tid]) s_base[tid] = atomicAdd(&count[tid], s_n[tid]);
+    if (tid == 2 && s_wmax) atomicMax(&count[2], s_wmax);
+    __syncthreads();
+    if (kind == 0) list[s_base[0] + slot] = (int)i;
+    else if (kind == 1) list[n - 1 - (s_base[1] + slot)] = (int)i;
 }
 
 // Cell-centre tables of a grid: colx[c] = cell_x(c), rowy[r] = cell_y(r).
