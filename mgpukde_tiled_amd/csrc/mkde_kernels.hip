@@ -317,8 +317,9 @@ MKDE_PART4(12, 13, 14, 15)
 
 // ======================================================== density raster ====
 //
-// One 256-thread workgroup computes a 64x64-cell block of a tile.  Lane l of
-// wave w owns column c0+l and rows r0+16w .. r0+16w+15 (16 accumulators).
+// One 256-thread workgroup computes a 64x64-cell block of a tile.  Each wave
+// owns a 32x32 quadrant (DT_SQUARE below), each lane one column of it and
+// 16 of its rows (16 accumulators).
 // Blocks without a valid cell only copy the mask's nodata values.
 // The workgroup walks the candidate bins, classifies every point of those
 // bins against the block exactly (EMPTY: no cell inside the cut-off; FULL:
@@ -327,7 +328,7 @@ MKDE_PART4(12, 13, 14, 15)
 //   1. cooperatively, in LDS: Ex[p][col] = exp2(dx2 a_p) for the 64 columns,
 //      Ey[p][row] = coef_p exp2(dy2 a_p) for the 64 rows, and for PARTIAL
 //      points the exact reach of each row (row_reach) and the class of each
-//      wave's 64x16 sub-block (the same exact EMPTY / FULL test on the
+//      wave's sub-block (the same exact EMPTY / FULL test on the
 //      smaller rectangle);
 //   2. each wave adds Ex * Ey[p][row] to its 16 cells with one FMA each
 //      (FULL), or with Ex replaced by 0 on the lanes with |dx| >= reach[p][row]
@@ -349,6 +350,30 @@ MKDE_PART4(12, 13, 14, 15)
 // [3] FULL / [4] PARTIAL / [5] EMPTY wave sub-blocks of PARTIAL points,
 // [6] blocks skipped (no valid cell)
 #define DT_NSTATS 8
+
+// Lane -> cells of a wave.  DT_SQUARE 1: wave w covers the 32x32 quadrant
+// starting at column 32 (w & 1), row 32 (w >> 1); lane l holds column
+// 32 (w & 1) + (l & 31) and rows 32 (w >> 1) + 16 (l >> 5) + j, j < 16 (the
+// DPP broadcast of Ey by 16-lane rows still applies).  DT_SQUARE 0: wave w
+// covers rows 16 w .. 16 w + 15 of all 64 columns.
+#ifndef DT_SQUARE
+#define DT_SQUARE 1
+#endif
+#if DT_SQUARE
+#define DT_WC(w, l) (32 * ((w) & 1) + ((l) & 31))
+#define DT_WR(w, l) (32 * ((w) >> 1) + 16 * ((l) >> 5))
+#define DT_WX0(w) (32 * ((w) & 1))
+#define DT_WNX 32
+#define DT_WY0(w) (32 * ((w) >> 1))
+#define DT_WNY 32
+#else
+#define DT_WC(w, l) (l)
+#define DT_WR(w, l) (16 * (w))
+#define DT_WX0(w) 0
+#define DT_WNX 64
+#define DT_WY0(w) (16 * (w))
+#define DT_WNY 16
+#endif
 
 // Raster of one or more tiles.  Single tile: g holds the tile geometry.
 // Tiled (tl.tw > 0): g.ncols / g.nrows are the raster size and each tile takes
@@ -440,6 +465,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     __shared__ __attribute__((aligned(16))) float s_xr[DT_SB * DT_EYP];   // row reach, PARTIAL points
     __shared__ float s_ccx[64], s_ccy[64];   // cell-centre x of the block's columns, y of its rows
     __shared__ float s_wyb[8];               // y of the first / last row of each wave's sub-block
+    __shared__ float s_wxb[8];               // x of the first / last column of each wave's sub-block
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const GridDesc g = A.g;
@@ -460,14 +486,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int64_t c0 = B.c0, r0 = B.r0, c1 = B.c1, r1 = B.r1;
     // tile-local indices for cell centres
     const int64_t lc0 = c0 - B.xoff, lr0 = r0 - B.yoff, lc1 = c1 - B.xoff, lr1 = r1 - B.yoff;
-    const int64_t col = c0 + lane;
+    const int wc = DT_WC(wave, lane), wr = DT_WR(wave, lane);   // this lane's column, first row
+    const int64_t col = c0 + wc;
 
     // validity of this thread's cells; blocks without a valid cell are skipped
     unsigned vbits = 0;
     if (col <= c1) {
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-            const int64_t row = r0 + wave * 16 + j;
+            const int64_t row = r0 + wr + j;
             if (row <= r1) {
                 bool v = true;
                 if (A.mask) v = A.mask[row * g.ncols + col] != g.nodata;
@@ -479,7 +506,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         if (col <= c1) {
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
-                const int64_t row = r0 + wave * 16 + j;
+                const int64_t row = r0 + wr + j;
                 if (row <= r1) A.out[row * g.ncols + col] = A.mask[row * g.ncols + col];
             }
         }
@@ -490,14 +517,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     // Block extent in cell-centre coordinates (exact N2 floats).
     const float bx0 = cell_x(xll, cs, lc0), bx1 = cell_x(xll, cs, lc1);
     const float by1 = cell_y(yll, cs, nrm05, lr0), by0 = cell_y(yll, cs, nrm05, lr1);
-    const float cx = cell_x(xll, cs, lc0 + lane);
-    if (tid < 64) s_ccx[tid] = cx;
+    const float cx = cell_x(xll, cs, lc0 + wc);
+    if (tid < 64) s_ccx[tid] = cell_x(xll, cs, lc0 + tid);
     else if (tid < 128) s_ccy[tid - 64] = cell_y(yll, cs, nrm05, lr0 + (tid - 64));
     else if (tid < 136) {
         // wave w = (tid - 128) / 2: [0] its first row, [1] its last row
-        const int64_t wr0 = lr0 + 8 * (tid - 128 - (tid & 1));
-        const int64_t wr1 = (wr0 + 15 < lr1) ? wr0 + 15 : lr1;
+        const int w = (tid - 128) >> 1;
+        const int64_t wr0 = lr0 + DT_WY0(w);
+        const int64_t wr1 = (wr0 + DT_WNY - 1 < lr1) ? wr0 + DT_WNY - 1 : lr1;
         s_wyb[tid - 128] = (tid & 1) ? cell_y(yll, cs, nrm05, wr1) : cell_y(yll, cs, nrm05, wr0);
+    } else if (tid < 144) {
+        // wave w = (tid - 136) / 2: [0] its first column, [1] its last column
+        const int w = (tid - 136) >> 1;
+        const int64_t wc0 = lc0 + DT_WX0(w);
+        const int64_t wc1 = (wc0 + DT_WNX - 1 < lc1) ? wc0 + DT_WNX - 1 : lc1;
+        s_wxb[tid - 136] = (tid & 1) ? cell_x(xll, cs, wc1) : cell_x(xll, cs, wc0);
     }
 
     // FP32 partial sums (at most DT_FOLD sub-batches of terms) folded into
@@ -552,12 +586,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 int cl4 = 0x55;   // 2 bits per wave: 0 skip, 1 full, 2 partial
                 if (q >> 31) {
                     cl4 = 0;
-                    const float fx = far1(px, bx0, bx1), nx = gap1(px, bx0, bx1);
 #pragma unroll
                     for (int w = 0; w < 4; ++w) {
                         int cw = 0;
-                        if (lr0 + 16 * w <= lr1) {
+                        if (lr0 + DT_WY0(w) <= lr1 && lc0 + DT_WX0(w) <= lc1) {
                             const float wy1 = s_wyb[2 * w], wy0 = s_wyb[2 * w + 1];
+                            const float wx0 = s_wxb[2 * w], wx1 = s_wxb[2 * w + 1];
+                            const float fx = far1(px, wx0, wx1), nx = gap1(px, wx0, wx1);
                             if (sq2(nx, gap1(py, wy0, wy1)) < thr) cw = sq2(fx, far1(py, wy0, wy1)) < thr ? 1 : 2;
                         }
                         cl4 |= cw << (2 * w);
@@ -592,20 +627,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         // 2a. block-FULL points: one FMA per cell, no tests.  Ex of this
         //     lane's column from LDS, Ey of the wave's 16 rows as one VGPR
         //     (lane l: row l & 15) broadcast by DPP.
-        const int eyo = wave * 16 + (lane & 15);
+        const int eyo = wr + (lane & 15);
         {
             // two points per step: the second pair of loads is in flight
             // while the first point's 16 FMAs run
             int p = 0;
             for (; p + 1 < nfull; p += 2) {
-                const float ey0 = s_ey[p * DT_EYP + eyo], ex0 = s_ex[p * 64 + lane];
-                const float ey1 = s_ey[(p + 1) * DT_EYP + eyo], ex1 = s_ex[(p + 1) * 64 + lane];
+                const float ey0 = s_ey[p * DT_EYP + eyo], ex0 = s_ex[p * 64 + wc];
+                const float ey1 = s_ey[(p + 1) * DT_EYP + eyo], ex1 = s_ex[(p + 1) * 64 + wc];
                 fma16_bcast(acc, ey0, ex0);
                 fma16_bcast(acc, ey1, ex1);
                 if (STATS && lane == 0) st_wf += 2;
             }
             if (p < nfull) {
-                fma16_bcast(acc, s_ey[p * DT_EYP + eyo], s_ex[p * 64 + lane]);
+                fma16_bcast(acc, s_ey[p * DT_EYP + eyo], s_ex[p * 64 + wc]);
                 if (STATS && lane == 0) ++st_wf;
             }
         }
@@ -616,14 +651,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 if (STATS && lane == 0) ++st_we;
                 continue;
             }
-            const float ex = s_ex[p * 64 + lane];
+            const float ex = s_ex[p * 64 + wc];
             const float eyv = s_ey[p * DT_EYP + eyo];
             if (cls == 1) {
                 fma16_bcast(acc, eyv, ex);
                 if (STATS && lane == 0) ++st_wf;
             } else {
                 const float dx = cx - s_px[p];
-                const float4 *xr = (const float4 *)&s_xr[p * DT_EYP + wave * 16];
+                const float4 *xr = (const float4 *)&s_xr[p * DT_EYP + wr];
                 part4_0(acc[0], acc[1], acc[2], acc[3], xr[0], ex, eyv, dx);
                 part4_4(acc[4], acc[5], acc[6], acc[7], xr[1], ex, eyv, dx);
                 part4_8(acc[8], acc[9], acc[10], acc[11], xr[2], ex, eyv, dx);
@@ -734,11 +769,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     if (item.z > 1) {
         double *pp = A.partials + (int64_t)item.w * 4096;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) pp[(wave * 16 + j) * 64 + lane] = sum[j];
+        for (int j = 0; j < 16; ++j) pp[(wr + j) * 64 + wc] = sum[j];
     } else if (col <= c1) {
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-            const int64_t row = r0 + wave * 16 + j;
+            const int64_t row = r0 + wr + j;
             if (row <= r1) {
                 const int64_t ix = row * g.ncols + col;
                 A.out[ix] = ((vbits >> j) & 1u) ? (float)sum[j] : A.mask[ix];
