@@ -1399,23 +1399,44 @@ extern "C" int mkde_last_kernel_ms(mkde_ctx *c, const char *which, double *ms)
 // ------------------------------------------------------------------ jobs --
 
 namespace {
-struct PhaseTimer {
-    hipStream_t s;
-    hipEvent_t a = nullptr, b = nullptr;
-    double *acc;
-    explicit PhaseTimer(hipStream_t st, double *accum) : s(st), acc(accum)
+// Phase times of a job: events are recorded around each phase and read once
+// at the end of the job, so timing adds no host synchronisation.
+struct PhaseTimes {
+    struct Ev {
+        hipEvent_t a, b;
+        double *acc;
+    };
+    std::vector<Ev> ev;
+    void finish()
     {
-        (void)hipEventCreate(&a);
-        (void)hipEventCreate(&b);
-        (void)hipEventRecord(a, s);
+        for (const Ev &e : ev) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess && e.acc) *e.acc += ms;
+        }
+    }
+    ~PhaseTimes()
+    {
+        for (const Ev &e : ev) {
+            (void)hipEventDestroy(e.a);
+            (void)hipEventDestroy(e.b);
+        }
+    }
+};
+struct PhaseTimer {
+    PhaseTimes &T;
+    hipStream_t s;
+    PhaseTimes::Ev e{nullptr, nullptr, nullptr};
+    PhaseTimer(PhaseTimes &times, hipStream_t st, double *accum) : T(times), s(st)
+    {
+        e.acc = accum;
+        (void)hipEventCreate(&e.a);
+        (void)hipEventCreate(&e.b);
+        (void)hipEventRecord(e.a, s);
     }
     ~PhaseTimer()
     {
-        float ms = 0.0f;
-        (void)hipEventRecord(b, s);
-        if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess && acc) *acc += ms;
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
+        (void)hipEventRecord(e.b, s);
+        T.ev.push_back(e);
     }
 };
 }  // namespace
@@ -1431,12 +1452,13 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
     if (o->hoption == -1) CHK(need_h(c));
     mkde_job_stats S;
     memset(&S, 0, sizeof S);
+    PhaseTimes times;
     hipEvent_t t0, t1;
     HIPCHK(hipEventCreate(&t0));
     HIPCHK(hipEventCreate(&t1));
     HIPCHK(hipEventRecord(t0, c->stream));
     if (ec_on) {
-        PhaseTimer pt(c->stream, &S.ms_boundary);
+        PhaseTimer pt(times, c->stream, &S.ms_boundary);
         if (o->recompute_boundary) {
             c->bflags_ok = false;
             c->nboundary = -1;
@@ -1453,7 +1475,7 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
     if (o->hoption != -1) CHK(mkde_set_bandwidth_fixed(c, S.h0));
     unsigned long long ne = 0;
     auto ec_pass = [&](void) -> int {
-        PhaseTimer pt(c->stream, &S.ms_ec);
+        PhaseTimer pt(times, c->stream, &S.ms_ec);
         int s = run_ec(c, c->h.p, c->h_fixed, c->ew.p, &ne);
         c->pts_dirty = true;
         return s;
@@ -1470,7 +1492,7 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
     if (o->hoption == 0 || o->hoption == -1) CHK(final_ec());
     if (o->hoption == 1) {
         {
-            PhaseTimer pt(c->stream, &S.ms_bandwidth);
+            PhaseTimer pt(times, c->stream, &S.ms_bandwidth);
             int it = 0;
             CHK(mkde_mle_fixed_bandwidth(c, S.h0, ec_on, &S.h_opt, &it));
         }
@@ -1479,7 +1501,7 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
     } else if (o->hoption == 2) {
         float h0 = S.h0, a0 = o->optimise ? -0.5f : o->alpha;
         {
-            PhaseTimer pt(c->stream, &S.ms_bandwidth);
+            PhaseTimer pt(times, c->stream, &S.ms_bandwidth);
             if (o->optimise) {
                 float out3[3];
                 int it = 0;
@@ -1497,11 +1519,12 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
     }
     S.n_ec_pairs = (int64_t)ne;
     {
-        PhaseTimer pt(c->stream, &S.ms_density);
+        PhaseTimer pt(times, c->stream, &S.ms_density);
         CHK(mkde_density_tile_resident(c));
     }
     HIPCHK(hipEventRecord(t1, c->stream));
     HIPCHK(hipEventSynchronize(t1));
+    times.finish();
     float tot = 0.0f;
     HIPCHK(hipEventElapsedTime(&tot, t0, t1));
     HIPCHK(hipEventElapsedTime(&c->last_density_ms, c->ev0, c->ev1));
