@@ -771,7 +771,8 @@ static int need_h(mkde_ctx *c)
 
 // EC weights for bandwidth array h (original order) or the fixed value hf > 0,
 // written to ew_out (original order).
-static int run_ec(mkde_ctx *c, const float *h, float hf, float *ew_out, unsigned long long *n_eval)
+static int run_ec(mkde_ctx *c, const float *h, float hf, float *ew_out, unsigned long long *n_eval,
+                  bool fold_count = false)
 {
     CHK(ensure_dist(c));
     CHK(c->list.ensure(c->n));
@@ -789,10 +790,10 @@ static int run_ec(mkde_ctx *c, const float *h, float hf, float *ew_out, unsigned
     HIPCHK(mkde_dev::ec(c->stream, c->n, cnt3[0], cnt3[1], cnt3[2], c->mg, R, c->colx.p, c->rowy.p, c->x.p, c->y.p,
                         h, hf, c->list.p, ew_out, c->ucount.p));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
-    if (n_eval) {
-        HIPCHK(mkde_dev::fold_slots(c->stream, c->ucount.p));
-        CHK(copy_d2h(c, n_eval, c->ucount.p + nw - 1, sizeof *n_eval));
-    }
+    // the pair count: folded, and read back unless the caller reads it later
+    // (fold_count: from the last word of ucount, after its own wait)
+    if (n_eval || fold_count) HIPCHK(mkde_dev::fold_slots(c->stream, c->ucount.p));
+    if (n_eval) CHK(copy_d2h(c, n_eval, c->ucount.p + nw - 1, sizeof *n_eval));
     HIPCHK(hipEventSynchronize(c->ev1));
     HIPCHK(hipEventElapsedTime(&c->last_ec_ms, c->ev0, c->ev1));
     return MKDE_OK;
@@ -1473,10 +1474,13 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
         S.h0 = rule_of_thumb_host(c->hx, c->hy, c->hw);
     }
     if (o->hoption != -1) CHK(mkde_set_bandwidth_fixed(c, S.h0));
-    unsigned long long ne = 0;
+    // the final EC pass's pair count and the geometric mean are read after
+    // the job's last wait (no extra host synchronisation)
+    bool ec_counted = false;
     auto ec_pass = [&](void) -> int {
         PhaseTimer pt(times, c->stream, &S.ms_ec);
-        int s = run_ec(c, c->h.p, c->h_fixed, c->ew.p, &ne);
+        int s = run_ec(c, c->h.p, c->h_fixed, c->ew.p, nullptr, true);
+        ec_counted = s == MKDE_OK;
         c->pts_dirty = true;
         return s;
     };
@@ -1514,10 +1518,8 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
         }
         S.h_opt = h0;
         S.alpha_opt = a0;
-        CHK(copy_d2h(c, &S.g, c->gval.p, sizeof(double)));
         CHK(final_ec());
     }
-    S.n_ec_pairs = (int64_t)ne;
     {
         PhaseTimer pt(times, c->stream, &S.ms_density);
         CHK(mkde_density_tile_resident(c));
@@ -1525,6 +1527,12 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
     HIPCHK(hipEventRecord(t1, c->stream));
     HIPCHK(hipEventSynchronize(t1));
     times.finish();
+    if (ec_counted) {
+        unsigned long long ne = 0;
+        CHK(copy_d2h(c, &ne, c->ucount.p + mkde_dev::slot_words() - 1, sizeof ne));
+        S.n_ec_pairs = (int64_t)ne;
+    }
+    if (o->hoption == 2) CHK(copy_d2h(c, &S.g, c->gval.p, sizeof(double)));
     float tot = 0.0f;
     HIPCHK(hipEventElapsedTime(&tot, t0, t1));
     HIPCHK(hipEventElapsedTime(&c->last_density_ms, c->ev0, c->ev1));
