@@ -103,52 +103,6 @@ __device__ __forceinline__ void kahan_fold(float part, float &sum, float &comp)
     sum = t;
 }
 
-// float-float arithmetic (compensated FP32).
-struct ff {
-    float hi, lo;
-};
-
-__device__ __forceinline__ ff ff_add(ff a, ff b)
-{
-    float s = a.hi + b.hi;
-    float bb = s - a.hi;
-    float e = (a.hi - (s - bb)) + (b.hi - bb);
-    e += a.lo + b.lo;
-    float r = s + e;
-    return ff{r, e - (r - s)};
-}
-
-__device__ __forceinline__ ff ff_add_f(ff a, float b)
-{
-    float s = a.hi + b;
-    float bb = s - a.hi;
-    float e = (a.hi - (s - bb)) + (b - bb);
-    e += a.lo;
-    float r = s + e;
-    return ff{r, e - (r - s)};
-}
-
-__device__ __forceinline__ ff ff_neg(ff a) { return ff{-a.hi, -a.lo}; }
-
-__device__ __forceinline__ ff ff_mul_f(ff a, float b)
-{
-    float p = a.hi * b;
-    float e = __builtin_fmaf(a.hi, b, -p);
-    e = __builtin_fmaf(a.lo, b, e);
-    float r = p + e;
-    return ff{r, e - (r - p)};
-}
-
-__device__ __forceinline__ ff ff_shfl_up(ff a, int d)
-{
-    return ff{__shfl_up(a.hi, d, WAVE), __shfl_up(a.lo, d, WAVE)};
-}
-
-__device__ __forceinline__ ff ff_shfl_xor(ff a, int d)
-{
-    return ff{__shfl_xor(a.hi, d, WAVE), __shfl_xor(a.lo, d, WAVE)};
-}
-
 // Order-preserving compaction across a 256-thread block.  Returns this
 // thread's slot among the flagged threads; *total gets the flagged count.
 __device__ __forceinline__ int block_compact256(bool flag, int *s_wcnt, int *total)
