@@ -761,7 +761,9 @@ __global__ __launch_bounds__(256) void k_block_cost(GridDesc g, const float *mas
     const BlockGeom B = block_geom(g, tl, nbx, blk);
     bool v = false;
     if (mask) {
-        for (int k = tid; k < 64 * 64; k += 256) {
+        // each thread stops at its first valid cell (most blocks are valid
+        // from their first row on)
+        for (int k = tid; k < 64 * 64 && !v; k += 256) {
             const int64_t r = B.r0 + k / 64, c = B.c0 + k % 64;
             if (r <= B.r1 && c <= B.c1 && mask[r * g.ncols + c] != g.nodata) v = true;
         }
