@@ -231,16 +231,27 @@ __device__ __forceinline__ void fma16_bcast(float (&a)[16], float eyv, float ex)
 // fl64(f^2 + dy2) >= thr.  f^2 is exact in FP64 and the sum is monotone in f,
 // so a cell of the row is inside the cut-off fl64(dx2 + dy2) < thr exactly
 // when |fl32(cx - px)| < f; f = 0 when no cell of the row is inside.  The
-// hardware FP64 square root (within about one FP32 ulp) only seeds the search;
-// the exact predicate then moves f by single ulps.
+// hardware FP64 square root (within about one FP32 ulp) only seeds the search.
+// The answer is almost always the seed or the float after it, which two
+// evaluations of the exact predicate settle without a loop: inside(seed)
+// says which of the two it can be, and the predicate at the neighbour on the
+// other side confirms it.  Anything else walks by single ulps.  v^2 is exact
+// in FP64 for a float v, so fma(v, v, dy2) rounds once, like fl64(v^2 + dy2).
 __device__ __forceinline__ float row_reach(double dy2, double thr)
 {
-    if (!(dy2 < thr)) return 0.0f;
-    auto inside = [&](float v) { return __dadd_rn((double)v * (double)v, dy2) < thr; };
-    float f = (float)__builtin_amdgcn_sqrt(thr - dy2);
-    for (int k = 0; k < 4096 && inside(f); ++k) f = __uint_as_float(__float_as_uint(f) + 1u);
-    for (int k = 0; k < 4096 && f > 0.0f && !inside(__uint_as_float(__float_as_uint(f) - 1u)); ++k)
-        f = __uint_as_float(__float_as_uint(f) - 1u);
+    auto inside = [&](float v) { return __fma_rn((double)v, (double)v, dy2) < thr; };
+    auto up = [](float v) { return __uint_as_float(__float_as_uint(v) + 1u); };
+    auto down = [](float v) { return __uint_as_float(__float_as_uint(v) - 1u); };
+    const bool any = dy2 < thr;
+    float f = (float)__builtin_amdgcn_sqrt(fmax(thr - dy2, 0.0));
+    // c0: the answer is above f; then f' = up(f) must be outside, else
+    // f' = f must have down(f) inside (f > 0: inside(0) holds when any)
+    const bool c0 = inside(f);
+    const float t = c0 ? up(f) : down(f);
+    const bool ok = f > 0.0f && (inside(t) != c0);
+    if (__builtin_expect(!any || ok, 1)) return any ? (c0 ? t : f) : 0.0f;
+    for (int k = 0; k < 4096 && inside(f); ++k) f = up(f);
+    for (int k = 0; k < 4096 && f > 0.0f && !inside(down(f)); ++k) f = down(f);
     return f;
 }
 
