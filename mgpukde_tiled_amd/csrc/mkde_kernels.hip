@@ -227,6 +227,20 @@ __device__ __forceinline__ void fma16_bcast(float (&a)[16], float eyv, float ex)
         : "v"(eyv), "v"(ex));
 }
 
+// Wait states between the matrix cores and the inline-assembly FMAs on the
+// same accumulators.  The compiler inserts them between its own instructions
+// but not around inline assembly, so both directions are pinned here: the
+// asm statements name the accumulators as operands, which keeps them between
+// the producer and the consumer.  A VALU write of a VGPR needs 2 wait states
+// before an MFMA reads it as SrcC (5 here); a 16-pass MFMA result needs 18
+// before a VALU instruction reads it (24 here).
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+#define MKDE_ACC16(a)                                                                                                 \
+    "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3]), "+v"(a[4]), "+v"(a[5]), "+v"(a[6]), "+v"(a[7]), "+v"(a[8]),     \
+        "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+__device__ __forceinline__ void wait_valu_to_mfma(float (&a)[16]) { asm volatile("s_nop 4" : MKDE_ACC16(a)); }
+__device__ __forceinline__ void wait_mfma_to_valu(float (&a)[16]) { asm volatile("s_nop 15\ns_nop 7" : MKDE_ACC16(a)); }
+
 // Reach of one raster row for one point: the smallest float f >= 0 with
 // fl64(f^2 + dy2) >= thr.  f^2 is exact in FP64 and the sum is monotone in f,
 // so a cell of the row is inside the cut-off fl64(dx2 + dy2) < thr exactly
@@ -316,29 +330,18 @@ MKDE_PART4(12, 13, 14, 15)
 // [6] blocks skipped (no valid cell)
 #define DT_NSTATS 8
 
-// Lane -> cells of a wave.  DT_SQUARE 1: wave w covers the 32x32 quadrant
-// starting at column 32 (w & 1), row 32 (w >> 1); lane l holds column
-// 32 (w & 1) + (l & 31) and rows 32 (w >> 1) + 16 (l >> 5) + j, j < 16 (the
-// DPP broadcast of Ey by 16-lane rows still applies).  DT_SQUARE 0: wave w
-// covers rows 16 w .. 16 w + 15 of all 64 columns.
-#ifndef DT_SQUARE
-#define DT_SQUARE 1
-#endif
-#if DT_SQUARE
+// Lane -> cells of a wave: wave w covers the 32x32 quadrant starting at
+// column 32 (w & 1), row 32 (w >> 1); lane l holds column 32 (w & 1) + (l & 31)
+// and, in accumulator j < 16, row 32 (w >> 1) + 8 (j >> 2) + 4 (l >> 5) + (j & 3):
+// the accumulator layout of v_mfma_f32_32x32x2_f32.  Accumulators 4g .. 4g+3
+// of a lane are four consecutive rows, and the lanes of one 16-lane DPP row
+// all hold the same 16 rows.
 #define DT_WC(w, l) (32 * ((w) & 1) + ((l) & 31))
-#define DT_WR(w, l) (32 * ((w) >> 1) + 16 * ((l) >> 5))
+#define DT_ROW(w, l, j) (32 * ((w) >> 1) + 8 * ((j) >> 2) + 4 * ((l) >> 5) + ((j) & 3))
 #define DT_WX0(w) (32 * ((w) & 1))
 #define DT_WNX 32
 #define DT_WY0(w) (32 * ((w) >> 1))
 #define DT_WNY 32
-#else
-#define DT_WC(w, l) (l)
-#define DT_WR(w, l) (16 * (w))
-#define DT_WX0(w) 0
-#define DT_WNX 64
-#define DT_WY0(w) (16 * (w))
-#define DT_WNY 16
-#endif
 
 // Raster of one or more tiles.  Single tile: g holds the tile geometry.
 // Tiled (tl.tw > 0): g.ncols / g.nrows are the raster size and each tile takes
@@ -451,7 +454,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int64_t c0 = B.c0, r0 = B.r0, c1 = B.c1, r1 = B.r1;
     // tile-local indices for cell centres
     const int64_t lc0 = c0 - B.xoff, lr0 = r0 - B.yoff, lc1 = c1 - B.xoff, lr1 = r1 - B.yoff;
-    const int wc = DT_WC(wave, lane), wr = DT_WR(wave, lane);   // this lane's column, first row
+    const int wc = DT_WC(wave, lane);   // this lane's column
     const int64_t col = c0 + wc;
 
     // validity of this thread's cells; blocks without a valid cell are skipped
@@ -459,7 +462,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     if (col <= c1) {
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-            const int64_t row = r0 + wr + j;
+            const int64_t row = r0 + DT_ROW(wave, lane, j);
             if (row <= r1) {
                 bool v = true;
                 if (A.mask) v = A.mask[row * g.ncols + col] != g.nodata;
@@ -471,7 +474,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         if (col <= c1) {
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
-                const int64_t row = r0 + wr + j;
+                const int64_t row = r0 + DT_ROW(wave, lane, j);
                 if (row <= r1) A.out[row * g.ncols + col] = A.mask[row * g.ncols + col];
             }
         }
@@ -589,26 +592,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             }
         }
         __syncthreads();
-        // 2a. block-FULL points: one FMA per cell, no tests.  Ex of this
-        //     lane's column from LDS, Ey of the wave's 16 rows as one VGPR
-        //     (lane l: row l & 15) broadcast by DPP.
-        const int eyo = wr + (lane & 15);
-        {
-            // two points per step: the second pair of loads is in flight
-            // while the first point's 16 FMAs run
+        // 2a. block-FULL points on the matrix cores: the quadrant's cells
+        //     += Ey (32 rows x 2 points) * Ex (2 points x 32 columns), one
+        //     v_mfma_f32_32x32x2_f32 per two points; lane l feeds row / column
+        //     l & 31 of point p + (l >> 5), and 0 past the last FULL point.
+        if (nfull > 0) {
+            wait_valu_to_mfma(acc);
+            f32x16 c;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) c[j] = acc[j];
+            const int k = lane >> 5, li = lane & 31;
+            const float *eyp = s_ey + DT_WY0(wave) + li, *exp = s_ex + DT_WX0(wave) + li;
+            // four points per step (the second pair's operands arrive while
+            // the first pair's MFMA runs), then the last one to three
             int p = 0;
-            for (; p + 1 < nfull; p += 2) {
-                const float ey0 = s_ey[p * DT_EYP + eyo], ex0 = s_ex[p * 64 + wc];
-                const float ey1 = s_ey[(p + 1) * DT_EYP + eyo], ex1 = s_ex[(p + 1) * 64 + wc];
-                fma16_bcast(acc, ey0, ex0);
-                fma16_bcast(acc, ey1, ex1);
-                if (STATS && lane == 0) st_wf += 2;
+            for (; p + 4 <= nfull; p += 4) {
+                const float ey0 = eyp[(p + k) * DT_EYP], ex0 = exp[(p + k) * 64];
+                const float ey1 = eyp[(p + 2 + k) * DT_EYP], ex1 = exp[(p + 2 + k) * 64];
+                c = __builtin_amdgcn_mfma_f32_32x32x2f32(ey0, ex0, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_32x32x2f32(ey1, ex1, c, 0, 0, 0);
             }
-            if (p < nfull) {
-                fma16_bcast(acc, s_ey[p * DT_EYP + eyo], s_ex[p * 64 + wc]);
-                if (STATS && lane == 0) ++st_wf;
+            for (; p < nfull; p += 2) {
+                const int q = p + k;   // <= nfull <= DT_SB - 1 when nfull is odd
+                const float ey = eyp[q * DT_EYP], ex = exp[q * 64];
+                c = __builtin_amdgcn_mfma_f32_32x32x2f32(q < nfull ? ey : 0.0f, q < nfull ? ex : 0.0f, c, 0, 0, 0);
             }
+#pragma unroll
+            for (int j = 0; j < 16; ++j) acc[j] = c[j];
+            wait_mfma_to_valu(acc);
+            if (STATS && lane == 0) st_wf += nfull;
         }
+        // Ey of the wave's rows as one VGPR: lane l holds the row of
+        // accumulator l & 15, broadcast to its 16-lane DPP row
+        const int eyo = DT_ROW(wave, lane, lane & 15);
         // 2b. block-PARTIAL points: per-wave class
         for (int p = nfull; p < cnt; ++p) {
             const int cls = (__builtin_amdgcn_readfirstlane(s_wcls[p]) >> (2 * wave)) & 3;
@@ -623,11 +639,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 if (STATS && lane == 0) ++st_wf;
             } else {
                 const float dx = cx - s_px[p];
-                const float4 *xr = (const float4 *)&s_xr[p * DT_EYP + wr];
+                // rows of accumulators 4g .. 4g+3: a float4 every 8 rows
+                const float4 *xr = (const float4 *)&s_xr[p * DT_EYP + DT_ROW(wave, lane, 0)];
                 part4_0(acc[0], acc[1], acc[2], acc[3], xr[0], ex, eyv, dx);
-                part4_4(acc[4], acc[5], acc[6], acc[7], xr[1], ex, eyv, dx);
-                part4_8(acc[8], acc[9], acc[10], acc[11], xr[2], ex, eyv, dx);
-                part4_12(acc[12], acc[13], acc[14], acc[15], xr[3], ex, eyv, dx);
+                part4_4(acc[4], acc[5], acc[6], acc[7], xr[2], ex, eyv, dx);
+                part4_8(acc[8], acc[9], acc[10], acc[11], xr[4], ex, eyv, dx);
+                part4_12(acc[12], acc[13], acc[14], acc[15], xr[6], ex, eyv, dx);
                 if (STATS && lane == 0) ++st_wp;
             }
         }
@@ -734,11 +751,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     if (item.z > 1) {
         double *pp = A.partials + (int64_t)item.w * 4096;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) pp[(wr + j) * 64 + wc] = sum[j];
+        for (int j = 0; j < 16; ++j) pp[DT_ROW(wave, lane, j) * 64 + wc] = sum[j];
     } else if (col <= c1) {
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-            const int64_t row = r0 + wr + j;
+            const int64_t row = r0 + DT_ROW(wave, lane, j);
             if (row <= r1) {
                 const int64_t ix = row * g.ncols + col;
                 A.out[ix] = ((vbits >> j) & 1u) ? (float)sum[j] : A.mask[ix];
