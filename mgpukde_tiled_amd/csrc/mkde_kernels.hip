@@ -207,26 +207,6 @@ __global__ __launch_bounds__(256) void k_bin_stats(int nbins, const int *bin_sta
     }
 }
 
-// ---- wave-uniform row values by DPP broadcast
-//
-// Ey of a wave's 16 rows lives in one VGPR: lane l holds row (l & 15), so
-// every 16-lane row of the wave carries all 16 values and DPP row_newbcast:j
-// hands row j's value to all 64 lanes as the FMA's first operand.  This
-// replaces a 64-lane LDS broadcast read per value.  Leading s_nop 1 gives the
-// two wait states a DPP read needs after a VALU write of its operands.
-#define MKDE_FMA_BC(J) "v_fmac_f32_dpp %" #J ", %16, %17 row_newbcast:" #J " row_mask:0xf bank_mask:0xf\n"
-
-// acc[j] += Ey[j] * ex, j = 0..15
-__device__ __forceinline__ void fma16_bcast(float (&a)[16], float eyv, float ex)
-{
-    asm("s_nop 1\n" MKDE_FMA_BC(0) MKDE_FMA_BC(1) MKDE_FMA_BC(2) MKDE_FMA_BC(3) MKDE_FMA_BC(4) MKDE_FMA_BC(5)
-            MKDE_FMA_BC(6) MKDE_FMA_BC(7) MKDE_FMA_BC(8) MKDE_FMA_BC(9) MKDE_FMA_BC(10) MKDE_FMA_BC(11)
-                MKDE_FMA_BC(12) MKDE_FMA_BC(13) MKDE_FMA_BC(14) MKDE_FMA_BC(15)
-        : "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3]), "+v"(a[4]), "+v"(a[5]), "+v"(a[6]), "+v"(a[7]), "+v"(a[8]),
-          "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
-        : "v"(eyv), "v"(ex));
-}
-
 // Wait states between the matrix cores and the inline-assembly FMAs on the
 // same accumulators.  The compiler inserts them between its own instructions
 // but not around inline assembly, so both directions are pinned here: the
@@ -269,11 +249,15 @@ __device__ __forceinline__ float row_reach(double dy2, double thr)
     return f;
 }
 
-// Rows R..R+3 of a PARTIAL sub-block: m_i = |dx| < reach_i ? ex : 0 (the
-// exact cut-off, row_reach), then acc[R+i] += Ey[R+i] * m_i with Ey broadcast
-// by DPP.  The four lane masks go to four SGPR pairs and the four m_i are
-// formed before the four DPP FMAs, so each m_i is at least 2 VALU
-// instructions old when it is read.
+// Accumulators R..R+3 of a PARTIAL sub-block: m_i = |dx| < reach_i ? ex : 0
+// (the exact cut-off, row_reach), then acc[R+i] += Ey[R+i] * m_i.  Ey of a
+// wave's 16 accumulator rows lives in one VGPR (lane l holds the row of
+// accumulator l & 15, the same 16 rows in every 16-lane row of the wave), and
+// DPP row_newbcast:j hands accumulator j's value to the lanes as the FMA's
+// first operand instead of a 64-lane LDS broadcast read per value.  The four
+// lane masks go to four SGPR pairs and the four m_i are formed before the
+// four DPP FMAs, so each m_i is at least 2 VALU instructions old when it is
+// read (the wait states a DPP read needs after a VALU write).
 #define MKDE_XCMP(I) "v_cmp_gt_f32_e64 %[s" #I "], %[x" #I "], |%[dx]|\n"
 #define MKDE_XSEL(I) "v_cndmask_b32_e64 %[m" #I "], 0, %[ex], %[s" #I "]\n"
 #define MKDE_XFMA(I, J) "v_fmac_f32_dpp %[a" #I "], %[ey], %[m" #I "] row_newbcast:" #J " row_mask:0xf bank_mask:0xf\n"
@@ -592,19 +576,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             }
         }
         __syncthreads();
-        // 2a. block-FULL points on the matrix cores: the quadrant's cells
-        //     += Ey (32 rows x 2 points) * Ex (2 points x 32 columns), one
-        //     v_mfma_f32_32x32x2_f32 per two points; lane l feeds row / column
-        //     l & 31 of point p + (l >> 5), and 0 past the last FULL point.
-        if (nfull > 0) {
+        // 2a. this wave's FULL points on the matrix cores: the quadrant's
+        //     cells += Ey (32 rows x 2 points) * Ex (2 points x 32 columns),
+        //     one v_mfma_f32_32x32x2_f32 per two points, lane l feeding row /
+        //     column l & 31 of the pair's point l >> 5 (0 for a missing
+        //     second point).  FULL are the block-FULL points [0, nfull) and
+        //     the block-PARTIAL points whose class for this wave is FULL;
+        //     the classes come as two wave-uniform bit masks over the
+        //     sub-batch's PARTIAL points.
+        const int np = cnt - nfull;   // <= DT_SB
+        const int wcl = lane < np ? (s_wcls[nfull + lane] >> (2 * wave)) & 3 : 0;
+        const uint64_t mf = __ballot(wcl == 1);
+        uint64_t mp = __ballot(wcl == 2);
+        if (STATS && lane == 0) {
+            st_wf += nfull + __popcll(mf);
+            st_wp += __popcll(mp);
+            st_we += np - __popcll(mf) - __popcll(mp);
+        }
+        if (nfull > 0 || mf) {
             wait_valu_to_mfma(acc);
             f32x16 c;
 #pragma unroll
             for (int j = 0; j < 16; ++j) c[j] = acc[j];
             const int k = lane >> 5, li = lane & 31;
             const float *eyp = s_ey + DT_WY0(wave) + li, *exp = s_ex + DT_WX0(wave) + li;
-            // four points per step (the second pair's operands arrive while
-            // the first pair's MFMA runs), then the last one to three
+            // block-FULL points four per step (the second pair's operands
+            // arrive while the first pair's MFMA runs), then the rest in pairs
             int p = 0;
             for (; p + 4 <= nfull; p += 4) {
                 const float ey0 = eyp[(p + k) * DT_EYP], ex0 = exp[(p + k) * 64];
@@ -612,41 +609,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 c = __builtin_amdgcn_mfma_f32_32x32x2f32(ey0, ex0, c, 0, 0, 0);
                 c = __builtin_amdgcn_mfma_f32_32x32x2f32(ey1, ex1, c, 0, 0, 0);
             }
-            for (; p < nfull; p += 2) {
-                const int q = p + k;   // <= nfull <= DT_SB - 1 when nfull is odd
-                const float ey = eyp[q * DT_EYP], ex = exp[q * 64];
-                c = __builtin_amdgcn_mfma_f32_32x32x2f32(q < nfull ? ey : 0.0f, q < nfull ? ex : 0.0f, c, 0, 0, 0);
+            uint64_t rest = (((1ull << (nfull - p)) - 1ull) << p) | (mf << nfull);
+            while (rest) {
+                const int p0 = __builtin_ctzll(rest);
+                rest &= rest - 1ull;
+                const int p1 = rest ? __builtin_ctzll(rest) : -1;
+                rest &= rest - 1ull;
+                const int q = k ? p1 : p0, qs = q < 0 ? 0 : q;
+                const float ey = eyp[qs * DT_EYP], ex = exp[qs * 64];
+                c = __builtin_amdgcn_mfma_f32_32x32x2f32(q >= 0 ? ey : 0.0f, q >= 0 ? ex : 0.0f, c, 0, 0, 0);
             }
 #pragma unroll
             for (int j = 0; j < 16; ++j) acc[j] = c[j];
             wait_mfma_to_valu(acc);
-            if (STATS && lane == 0) st_wf += nfull;
         }
-        // Ey of the wave's rows as one VGPR: lane l holds the row of
-        // accumulator l & 15, broadcast to its 16-lane DPP row
+        // 2b. this wave's PARTIAL points: Ey of the wave's rows as one VGPR
+        //     (lane l holds the row of accumulator l & 15, broadcast to its
+        //     16-lane DPP row), the FMAs masked by the row reach
         const int eyo = DT_ROW(wave, lane, lane & 15);
-        // 2b. block-PARTIAL points: per-wave class
-        for (int p = nfull; p < cnt; ++p) {
-            const int cls = (__builtin_amdgcn_readfirstlane(s_wcls[p]) >> (2 * wave)) & 3;
-            if (cls == 0) {
-                if (STATS && lane == 0) ++st_we;
-                continue;
-            }
+        while (mp) {
+            const int p = nfull + __builtin_ctzll(mp);
+            mp &= mp - 1ull;
             const float ex = s_ex[p * 64 + wc];
             const float eyv = s_ey[p * DT_EYP + eyo];
-            if (cls == 1) {
-                fma16_bcast(acc, eyv, ex);
-                if (STATS && lane == 0) ++st_wf;
-            } else {
-                const float dx = cx - s_px[p];
-                // rows of accumulators 4g .. 4g+3: a float4 every 8 rows
-                const float4 *xr = (const float4 *)&s_xr[p * DT_EYP + DT_ROW(wave, lane, 0)];
-                part4_0(acc[0], acc[1], acc[2], acc[3], xr[0], ex, eyv, dx);
-                part4_4(acc[4], acc[5], acc[6], acc[7], xr[2], ex, eyv, dx);
-                part4_8(acc[8], acc[9], acc[10], acc[11], xr[4], ex, eyv, dx);
-                part4_12(acc[12], acc[13], acc[14], acc[15], xr[6], ex, eyv, dx);
-                if (STATS && lane == 0) ++st_wp;
-            }
+            const float dx = cx - s_px[p];
+            // rows of accumulators 4g .. 4g+3: a float4 every 8 rows
+            const float4 *xr = (const float4 *)&s_xr[p * DT_EYP + DT_ROW(wave, lane, 0)];
+            part4_0(acc[0], acc[1], acc[2], acc[3], xr[0], ex, eyv, dx);
+            part4_4(acc[4], acc[5], acc[6], acc[7], xr[2], ex, eyv, dx);
+            part4_8(acc[8], acc[9], acc[10], acc[11], xr[4], ex, eyv, dx);
+            part4_12(acc[12], acc[13], acc[14], acc[15], xr[6], ex, eyv, dx);
         }
         if (++npend == DT_FOLD) fold();
         __syncthreads();
