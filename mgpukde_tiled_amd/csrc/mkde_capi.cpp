@@ -1234,9 +1234,9 @@ extern "C" int mkde_raster_upload(mkde_ctx *c, int64_t ncols, int64_t nrows, con
     if (tile_rows > nrows) tile_rows = nrows;
     mkde_grid rg{ncols, nrows, (float)gt[0], (float)(gt[3] + (double)nrows * gt[5]), (float)gt[1], nodata};
     CHK(check_grid(&rg));
-    const int64_t bpx = (tile_cols + 63) / 64, bpy = (tile_rows + 63) / 64;
+    const int64_t bpx = (tile_cols + MKDE_DT_BW - 1) / MKDE_DT_BW, bpy = (tile_rows + 63) / 64;
     const int64_t ntx = (ncols + tile_cols - 1) / tile_cols, nty = (nrows + tile_rows - 1) / tile_rows;
-    if (bpx * bpy * ntx * nty > INT32_MAX) return fail(MKDE_E_RANGE, "raster has more than 2^31-1 blocks of 64x64");
+    if (bpx * bpy * ntx * nty > INT32_MAX) return fail(MKDE_E_RANGE, "raster has more than 2^31-1 density blocks");
     CHK(mkde_tile_upload(c, &rg, mask));
     c->tl = Tiling{tile_cols, tile_rows, (int)ntx, (int)bpx, (int)(bpx * bpy), gt[0], gt[1], gt[3], gt[5]};
     return MKDE_OK;
@@ -1274,7 +1274,7 @@ extern "C" int mkde_density_tile_resident(mkde_ctx *c)
         CHK(c->dt_items.ensure(n_items));
         CHK(c->dt_order.ensure(n_items));
         CHK(c->dt_splits.ensure(n_split));
-        CHK(c->dt_partials.ensure((size_t)cnt[1] * 4096));
+        CHK(c->dt_partials.ensure((size_t)cnt[1] * MKDE_DT_CELLS));
         HIPCHK(mkde_dev::density_items(c->stream, (int)nblk, W, n_items, c->dt_items.p, c->dt_splits.p,
                                        c->dt_order.p));
         order = c->dt_order.p;

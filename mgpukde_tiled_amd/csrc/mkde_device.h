@@ -1,5 +1,5 @@
 // mkde_device.h -- types shared by the HIP kernels (mkde_kernels.hip) and the
-// host library (mkde_capi.cpp).  Plain structs only: no torch, no STL.
+// host library (mkde_capi.cpp, mkde_serve.cpp).  Plain structs only: no torch, no STL.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -50,10 +50,15 @@ struct PointIndex {
     int64_t n;
 };
 
+// Density blocks: 64 rows x MKDE_DT_BW columns (64 or 128), one workgroup
+// each, one 32 x 32 quadrant per wave (MKDE_DT_BW / 16 waves).
+#define MKDE_DT_BW 64
+#define MKDE_DT_CELLS (64 * MKDE_DT_BW)
+
 // A raster processed as tiles of tw x th cells, each tile with its own origin
-// from the raster geotransform (GDAL order gt0, gt1, gt3, gt5).  Blocks of
-// 64 x 64 cells are numbered tile-major: bpx blocks per tile row, bpt per
-// tile, ntx tiles per raster row (edge tiles leave some blocks empty).
+// from the raster geotransform (GDAL order gt0, gt1, gt3, gt5).  Density
+// blocks are numbered tile-major: bpx blocks per tile row, bpt per tile, ntx
+// tiles per raster row (edge tiles leave some blocks empty).
 // tw == 0 means one tile described by its GridDesc.
 struct Tiling {
     int64_t tw, th;
@@ -61,7 +66,7 @@ struct Tiling {
     double gt0, gt1, gt3, gt5;
 };
 
-// Device buffers of the density kernel's work list (per 64x64 block unless
+// Device buffers of the density kernel's work list (per density block unless
 // noted): candidate counts, split parts, scans of parts / split parts / split
 // flags, scan scratch, the candidate total and the {items, slots, splits}
 // counts read back by the host.

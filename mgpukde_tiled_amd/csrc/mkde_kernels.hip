@@ -103,21 +103,29 @@ __device__ __forceinline__ void kahan_fold(float part, float &sum, float &comp)
     sum = t;
 }
 
-// Order-preserving compaction across a 256-thread block.  Returns this
+// Order-preserving compaction across a workgroup of NW waves.  Returns this
 // thread's slot among the flagged threads; *total gets the flagged count.
-__device__ __forceinline__ int block_compact256(bool flag, int *s_wcnt, int *total)
+template <int NW>
+__device__ __forceinline__ int block_compact(bool flag, int *s_wcnt, int *total)
 {
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
     uint64_t m = __ballot(flag);
     int rank = __popcll(m & ((1ull << lane) - 1ull));
     if (lane == 0) s_wcnt[wave] = __popcll(m);
     __syncthreads();
-    int off = 0;
+    int off = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < 4; ++w) off += (w < wave) ? s_wcnt[w] : 0;
-    *total = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
+    for (int w = 0; w < NW; ++w) {
+        off += (w < wave) ? s_wcnt[w] : 0;
+        tot += s_wcnt[w];
+    }
+    *total = tot;
     __syncthreads();
     return off + rank;
+}
+__device__ __forceinline__ int block_compact256(bool flag, int *s_wcnt, int *total)
+{
+    return block_compact<4>(flag, s_wcnt, total);
 }
 
 __device__ __forceinline__ int clampi(int64_t v, int64_t lo, int64_t hi)
@@ -280,9 +288,9 @@ MKDE_PART4(12, 13, 14, 15)
 
 // ======================================================== density raster ====
 //
-// One 256-thread workgroup computes a 64x64-cell block of a tile.  Each wave
-// owns a 32x32 quadrant (DT_SQUARE below), each lane one column of it and
-// 16 of its rows (16 accumulators).
+// One workgroup computes a block of 64 x MKDE_DT_BW cells of a tile (64 x 64,
+// 256 threads).  Each wave owns a 32x32 quadrant (lane layout below), each
+// lane one column of it and 16 of its rows (16 accumulators).
 // Blocks without a valid cell only copy the mask's nodata values.
 // The workgroup walks the candidate bins, classifies every point of those
 // bins against the block exactly (EMPTY: no cell inside the cut-off; FULL:
@@ -304,7 +312,7 @@ MKDE_PART4(12, 13, 14, 15)
 
 #define DT_SB 32
 #define DT_EYP 68   // floats per point row of the Ey and reach tables (64 + pad)
-#define DT_Q (DT_SB + 256)
+#define DT_Q (DT_SB + DT_NT)
 // Sub-batches summed in FP32 before the fold into FP64: at most 64 positive
 // terms, a relative error below 64 * 2^-24 = 3.8e-6 per fold
 #define DT_FOLD 2
@@ -314,17 +322,21 @@ MKDE_PART4(12, 13, 14, 15)
 // [6] blocks skipped (no valid cell)
 #define DT_NSTATS 8
 
-// Lane -> cells of a wave: wave w covers the 32x32 quadrant starting at
-// column 32 (w & 1), row 32 (w >> 1); lane l holds column 32 (w & 1) + (l & 31)
-// and, in accumulator j < 16, row 32 (w >> 1) + 8 (j >> 2) + 4 (l >> 5) + (j & 3):
+// Lane -> cells of a wave: the block's 64 x MKDE_DT_BW cells are DT_QX x 2
+// quadrants of 32 x 32, wave w covering the one at column 32 (w % DT_QX), row
+// 32 (w / DT_QX); lane l holds column 32 (w % DT_QX) + (l & 31) and, in
+// accumulator j < 16, row 32 (w / DT_QX) + 8 (j >> 2) + 4 (l >> 5) + (j & 3):
 // the accumulator layout of v_mfma_f32_32x32x2_f32.  Accumulators 4g .. 4g+3
 // of a lane are four consecutive rows, and the lanes of one 16-lane DPP row
 // all hold the same 16 rows.
-#define DT_WC(w, l) (32 * ((w) & 1) + ((l) & 31))
-#define DT_ROW(w, l, j) (32 * ((w) >> 1) + 8 * ((j) >> 2) + 4 * ((l) >> 5) + ((j) & 3))
-#define DT_WX0(w) (32 * ((w) & 1))
+#define DT_QX (MKDE_DT_BW / 32)
+#define DT_NW (2 * DT_QX)   // waves per workgroup
+#define DT_NT (64 * DT_NW)  // threads per workgroup
+#define DT_WX0(w) (32 * ((w) % DT_QX))
+#define DT_WY0(w) (32 * ((w) / DT_QX))
+#define DT_WC(w, l) (DT_WX0(w) + ((l) & 31))
+#define DT_ROW(w, l, j) (DT_WY0(w) + 8 * ((j) >> 2) + 4 * ((l) >> 5) + ((j) & 3))
 #define DT_WNX 32
-#define DT_WY0(w) (32 * ((w) >> 1))
 #define DT_WNY 32
 
 // Raster of one or more tiles.  Single tile: g holds the tile geometry.
@@ -348,8 +360,8 @@ struct TileArgs {
     const int *order;            // nullable: items in raster order, one per block
     const int4 *items;           // with order: work items
     const unsigned *cost;        // with order: candidate count of each block
-    double *partials;            // 4096 per slot
-    int nbx;                     // 64-cell blocks per raster row
+    double *partials;            // MKDE_DT_CELLS per slot
+    int nbx;                     // density blocks per raster row
     Tiling tl;
 };
 
@@ -369,9 +381,9 @@ __device__ __forceinline__ BlockGeom block_geom(const GridDesc &g, const Tiling 
         b.yoff = (int64_t)(t / tl.ntx) * tl.th;
         const int64_t tcols = (tl.tw < g.ncols - b.xoff) ? tl.tw : g.ncols - b.xoff;
         const int64_t trows = (tl.th < g.nrows - b.yoff) ? tl.th : g.nrows - b.yoff;
-        b.c0 = b.xoff + (int64_t)(lb % tl.bpx) * 64;
+        b.c0 = b.xoff + (int64_t)(lb % tl.bpx) * MKDE_DT_BW;
         b.r0 = b.yoff + (int64_t)(lb / tl.bpx) * 64;
-        b.c1 = (b.c0 + 63 < b.xoff + tcols - 1) ? b.c0 + 63 : b.xoff + tcols - 1;
+        b.c1 = (b.c0 + MKDE_DT_BW - 1 < b.xoff + tcols - 1) ? b.c0 + MKDE_DT_BW - 1 : b.xoff + tcols - 1;
         b.r1 = (b.r0 + 63 < b.yoff + trows - 1) ? b.r0 + 63 : b.yoff + trows - 1;
         const double g0 = tl.gt0 + tl.gt1 * (double)b.xoff;
         const double g3 = tl.gt3 + tl.gt5 * (double)b.yoff;
@@ -380,9 +392,9 @@ __device__ __forceinline__ BlockGeom block_geom(const GridDesc &g, const Tiling 
         b.cs = (double)(float)tl.gt1;
         b.nrm05 = (double)trows - 0.5;
     } else {
-        b.c0 = (int64_t)(blk % nbx) * 64;
+        b.c0 = (int64_t)(blk % nbx) * MKDE_DT_BW;
         b.r0 = (int64_t)(blk / nbx) * 64;
-        b.c1 = (b.c0 + 63 < g.ncols - 1) ? b.c0 + 63 : g.ncols - 1;
+        b.c1 = (b.c0 + MKDE_DT_BW - 1 < g.ncols - 1) ? b.c0 + MKDE_DT_BW - 1 : g.ncols - 1;
         b.r1 = (b.r0 + 63 < g.nrows - 1) ? b.r0 + 63 : g.nrows - 1;
         b.xoff = b.yoff = 0;
         b.xll = g.xll;
@@ -393,13 +405,17 @@ __device__ __forceinline__ BlockGeom block_geom(const GridDesc &g, const Tiling 
     return b;
 }
 
-// 4 waves per SIMD: at most 128 VGPRs (no spills) and 37.8 KB of LDS per
-// workgroup, 4 workgroups per CU.
+// 4 waves per SIMD: at most 128 VGPRs (no spills).  64 x 64 blocks: 37.7 KB
+// of LDS per workgroup, 4 workgroups per CU, so each SIMD's 4 waves belong to
+// 4 workgroups and a barrier wait in one leaves 3 to issue.  (64 x 128
+// blocks share Ey and the reach over twice the columns, but at 55 KB and 8
+// waves only 2 workgroups fit per CU: measured 31.6 ms against 24.3 ms on
+// the adaptive 1M job.)
 template <bool STATS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_density_tile(TileArgs A)
+__global__ __launch_bounds__(DT_NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_density_tile(TileArgs A)
 {
-    __shared__ int s_bins[256];
-    __shared__ int s_wcnt[4];
+    __shared__ int s_bins[DT_NT];
+    __shared__ int s_wcnt[DT_NW];
     __shared__ unsigned s_queue[DT_Q];   // point index | 0x80000000 for block-PARTIAL
     // records of the queued points, stored when they are classified, so a
     // sub-batch reads its parameters from LDS instead of global memory
@@ -412,12 +428,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     __shared__ double s_thr[DT_SB];
     __shared__ int s_wcls[DT_SB];   // 2 bits per wave: 0 skip, 1 full, 2 partial
     __shared__ int s_nfull;         // block-FULL points at the front of the sub-batch
-    __shared__ __attribute__((aligned(16))) float s_ex[DT_SB * 64];
+    __shared__ __attribute__((aligned(16))) float s_ex[DT_SB * MKDE_DT_BW];
     __shared__ __attribute__((aligned(16))) float s_ey[DT_SB * DT_EYP];
     __shared__ __attribute__((aligned(16))) float s_xr[DT_SB * DT_EYP];   // row reach, PARTIAL points
-    __shared__ float s_ccx[64], s_ccy[64];   // cell-centre x of the block's columns, y of its rows
-    __shared__ float s_wyb[8];               // y of the first / last row of each wave's sub-block
-    __shared__ float s_wxb[8];               // x of the first / last column of each wave's sub-block
+    __shared__ float s_ccx[MKDE_DT_BW], s_ccy[64];   // cell-centre x of the block's columns, y of its rows
+    __shared__ float s_wyb[2 * DT_NW];               // y of the first / last row of each wave's sub-block
+    __shared__ float s_wxb[2 * DT_NW];               // x of the first / last column of each wave's sub-block
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const GridDesc g = A.g;
@@ -470,20 +486,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const float bx0 = cell_x(xll, cs, lc0), bx1 = cell_x(xll, cs, lc1);
     const float by1 = cell_y(yll, cs, nrm05, lr0), by0 = cell_y(yll, cs, nrm05, lr1);
     const float cx = cell_x(xll, cs, lc0 + wc);
-    if (tid < 64) s_ccx[tid] = cell_x(xll, cs, lc0 + tid);
-    else if (tid < 128) s_ccy[tid - 64] = cell_y(yll, cs, nrm05, lr0 + (tid - 64));
-    else if (tid < 136) {
-        // wave w = (tid - 128) / 2: [0] its first row, [1] its last row
-        const int w = (tid - 128) >> 1;
-        const int64_t wr0 = lr0 + DT_WY0(w);
-        const int64_t wr1 = (wr0 + DT_WNY - 1 < lr1) ? wr0 + DT_WNY - 1 : lr1;
-        s_wyb[tid - 128] = (tid & 1) ? cell_y(yll, cs, nrm05, wr1) : cell_y(yll, cs, nrm05, wr0);
-    } else if (tid < 144) {
-        // wave w = (tid - 136) / 2: [0] its first column, [1] its last column
-        const int w = (tid - 136) >> 1;
-        const int64_t wc0 = lc0 + DT_WX0(w);
-        const int64_t wc1 = (wc0 + DT_WNX - 1 < lc1) ? wc0 + DT_WNX - 1 : lc1;
-        s_wxb[tid - 136] = (tid & 1) ? cell_x(xll, cs, wc1) : cell_x(xll, cs, wc0);
+    {
+        // thread ranges (even starts): columns, rows, then per wave [0] first /
+        // [1] last row and [0] first / [1] last column of its quadrant
+        constexpr int T1 = MKDE_DT_BW, T2 = T1 + 64, T3 = T2 + 2 * DT_NW, T4 = T3 + 2 * DT_NW;
+        static_assert(T4 <= DT_NT, "too few threads for the block tables");
+        if (tid < T1) s_ccx[tid] = cell_x(xll, cs, lc0 + tid);
+        else if (tid < T2) s_ccy[tid - T1] = cell_y(yll, cs, nrm05, lr0 + (tid - T1));
+        else if (tid < T3) {
+            const int w = (tid - T2) >> 1;
+            const int64_t wr0 = lr0 + DT_WY0(w);
+            const int64_t wr1 = (wr0 + DT_WNY - 1 < lr1) ? wr0 + DT_WNY - 1 : lr1;
+            s_wyb[tid - T2] = (tid & 1) ? cell_y(yll, cs, nrm05, wr1) : cell_y(yll, cs, nrm05, wr0);
+        } else if (tid < T4) {
+            const int w = (tid - T3) >> 1;
+            const int64_t wc0 = lc0 + DT_WX0(w);
+            const int64_t wc1 = (wc0 + DT_WNX - 1 < lc1) ? wc0 + DT_WNX - 1 : lc1;
+            s_wxb[tid - T3] = (tid & 1) ? cell_x(xll, cs, wc1) : cell_x(xll, cs, wc0);
+        }
     }
 
     // FP32 partial sums (at most DT_FOLD sub-batches of terms) folded into
@@ -535,11 +555,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 s_a[t] = s_qa[e];
                 s_thr[t] = thr;
                 // wave sub-block classes
-                int cl4 = 0x55;   // 2 bits per wave: 0 skip, 1 full, 2 partial
+                // 2 bits per wave: 0 skip, 1 full, 2 partial
+                int cl4 = (int)(0x55555555u >> (32 - 2 * DT_NW));
                 if (q >> 31) {
                     cl4 = 0;
 #pragma unroll
-                    for (int w = 0; w < 4; ++w) {
+                    for (int w = 0; w < DT_NW; ++w) {
                         int cw = 0;
                         if (lr0 + DT_WY0(w) <= lr1 && lc0 + DT_WX0(w) <= lc1) {
                             const float wy1 = s_wyb[2 * w], wy0 = s_wyb[2 * w + 1];
@@ -554,23 +575,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             }
         }
         __syncthreads();
-        // 1b. tables: lane -> row / column slot, wave -> 8 points; consecutive
-        //     lanes write consecutive LDS words (no bank conflicts)
+        // 1b. tables: lane -> row / column slot, wave -> DT_SB / DT_NW points;
+        //     consecutive lanes write consecutive LDS words (no bank
+        //     conflicts).  Ey and the reach of a row serve every column of
+        //     the block, so a wider block shares them.
         const int nfull = s_nfull;
         {
-            const float cyr = s_ccy[lane], cxr = s_ccx[lane];
+            const float cyr = s_ccy[lane];
+            float cxr[MKDE_DT_BW / 64];
+#pragma unroll
+            for (int cb = 0; cb < MKDE_DT_BW / 64; ++cb) cxr[cb] = s_ccx[lane + 64 * cb];
             // the point index is wave-uniform: made explicit (scalar index
             // and bound test, fewer VALU address operations)
-            const int pw = __builtin_amdgcn_readfirstlane(wave) * (DT_SB / 4);
+            const int pw = __builtin_amdgcn_readfirstlane(wave) * (DT_SB / DT_NW);
 #pragma unroll 2
-            for (int q = 0; q < DT_SB / 4; ++q) {
+            for (int q = 0; q < DT_SB / DT_NW; ++q) {
                 const int p = pw + q;
                 if (p < cnt) {
                     const float py = s_py[p], px = s_px[p], coef = s_coef[p];
                     const double a = s_a[p];
                     const double dy2 = sqd(cyr - py);
                     s_ey[p * DT_EYP + lane] = fast_exp2((float)(dy2 * a)) * coef;
-                    s_ex[p * 64 + lane] = fast_exp2((float)(sqd(cxr - px) * a));
+#pragma unroll
+                    for (int cb = 0; cb < MKDE_DT_BW / 64; ++cb)
+                        s_ex[p * MKDE_DT_BW + lane + 64 * cb] = fast_exp2((float)(sqd(cxr[cb] - px) * a));
                     if (p >= nfull) s_xr[p * DT_EYP + lane] = row_reach(dy2, s_thr[p]);
                 }
             }
@@ -604,8 +632,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             // arrive while the first pair's MFMA runs), then the rest in pairs
             int p = 0;
             for (; p + 4 <= nfull; p += 4) {
-                const float ey0 = eyp[(p + k) * DT_EYP], ex0 = exp[(p + k) * 64];
-                const float ey1 = eyp[(p + 2 + k) * DT_EYP], ex1 = exp[(p + 2 + k) * 64];
+                const float ey0 = eyp[(p + k) * DT_EYP], ex0 = exp[(p + k) * MKDE_DT_BW];
+                const float ey1 = eyp[(p + 2 + k) * DT_EYP], ex1 = exp[(p + 2 + k) * MKDE_DT_BW];
                 c = __builtin_amdgcn_mfma_f32_32x32x2f32(ey0, ex0, c, 0, 0, 0);
                 c = __builtin_amdgcn_mfma_f32_32x32x2f32(ey1, ex1, c, 0, 0, 0);
             }
@@ -616,7 +644,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 const int p1 = rest ? __builtin_ctzll(rest) : -1;
                 rest &= rest - 1ull;
                 const int q = k ? p1 : p0, qs = q < 0 ? 0 : q;
-                const float ey = eyp[qs * DT_EYP], ex = exp[qs * 64];
+                const float ey = eyp[qs * DT_EYP], ex = exp[qs * MKDE_DT_BW];
                 c = __builtin_amdgcn_mfma_f32_32x32x2f32(q >= 0 ? ey : 0.0f, q >= 0 ? ex : 0.0f, c, 0, 0, 0);
             }
 #pragma unroll
@@ -630,7 +658,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         while (mp) {
             const int p = nfull + __builtin_ctzll(mp);
             mp &= mp - 1ull;
-            const float ex = s_ex[p * 64 + wc];
+            const float ex = s_ex[p * MKDE_DT_BW + wc];
             const float eyv = s_ey[p * DT_EYP + eyo];
             const float dx = cx - s_px[p];
             // rows of accumulators 4g .. 4g+3: a float4 every 8 rows
@@ -644,7 +672,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         __syncthreads();
     };
 
-    for (int base = 0; base < nb_total; base += 256) {
+    for (int base = 0; base < nb_total; base += DT_NT) {
         // candidate bins of this chunk
         const int k = base + tid;
         bool cand = false;
@@ -657,7 +685,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             }
         }
         int nb;
-        int slot = block_compact256(cand, s_wcnt, &nb);
+        int slot = block_compact<DT_NW>(cand, s_wcnt, &nb);
         if (cand) s_bins[slot] = b;
         __syncthreads();
         for (int kb = 0; kb < nb; ++kb) {
@@ -669,7 +697,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             if (o1 <= win0 || o0 >= win1) continue;
             const int st = bst + (int)(win0 > o0 ? win0 - o0 : 0);
             const int en = ben - (int)(o1 > win1 ? o1 - win1 : 0);
-            for (int off = st; off < en; off += 256) {
+            for (int off = st; off < en; off += DT_NT) {
                 const int i = off + tid;
                 int cls = 0;   // 0 empty, 1 full, 2 partial
                 PtRec p;
@@ -687,7 +715,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                     if (STATS && cls == 2) ++st_part;
                 }
                 int na;
-                int pos = block_compact256(cls != 0, s_wcnt, &na);
+                int pos = block_compact<DT_NW>(cls != 0, s_wcnt, &na);
                 if (cls != 0) {
                     const int e = qn + pos;
                     s_queue[e] = (unsigned)i | (cls == 2 ? 0x80000000u : 0u);
@@ -741,9 +769,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     // write back: nodata cells keep the mask value; parts of a split block
     // leave FP64 partial sums for k_density_merge
     if (item.z > 1) {
-        double *pp = A.partials + (int64_t)item.w * 4096;
+        double *pp = A.partials + (int64_t)item.w * MKDE_DT_CELLS;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) pp[DT_ROW(wave, lane, j) * 64 + wc] = sum[j];
+        for (int j = 0; j < 16; ++j) pp[DT_ROW(wave, lane, j) * MKDE_DT_BW + wc] = sum[j];
     } else if (col <= c1) {
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
@@ -769,7 +797,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     }
 }
 
-// Work estimate of each 64x64 block: points in the candidate bins (0 for
+// Work estimate of each density block: points in the candidate bins (0 for
 // blocks without a valid cell), then a heaviest-first visiting order by
 // log2 buckets, so the long blocks start early and the launch drains evenly.
 __global__ __launch_bounds__(256) void k_block_cost(GridDesc g, const float *mask, PointIndex I, int nbx,
@@ -783,8 +811,8 @@ __global__ __launch_bounds__(256) void k_block_cost(GridDesc g, const float *mas
     if (mask) {
         // each thread stops at its first valid cell (most blocks are valid
         // from their first row on)
-        for (int k = tid; k < 64 * 64 && !v; k += 256) {
-            const int64_t r = B.r0 + k / 64, c = B.c0 + k % 64;
+        for (int k = tid; k < MKDE_DT_CELLS && !v; k += 256) {
+            const int64_t r = B.r0 + k / MKDE_DT_BW, c = B.c0 + k % MKDE_DT_BW;
             if (r <= B.r1 && c <= B.c1 && mask[r * g.ncols + c] != g.nodata) v = true;
         }
     } else {
@@ -835,18 +863,16 @@ __global__ __launch_bounds__(256) void k_density_merge(GridDesc g, Tiling tl, in
                                                        const int4 *splits, const double *partials, float *out)
 {
     const int4 sp = splits[blockIdx.x];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const BlockGeom B = block_geom(g, tl, nbx, sp.x);
-    const int64_t col = B.c0 + lane;
-    if (col > B.c1) return;
-#pragma unroll 4
-    for (int j = 0; j < 16; ++j) {
-        const int64_t row = B.r0 + wave * 16 + j;
-        if (row > B.r1) break;
+    // 256 threads over the block's 64 x MKDE_DT_BW cells, row-major
+    for (int k = threadIdx.x; k < MKDE_DT_CELLS; k += 256) {
+        const int r = k / MKDE_DT_BW, cc = k % MKDE_DT_BW;
+        const int64_t row = B.r0 + r, col = B.c0 + cc;
+        if (row > B.r1 || col > B.c1) continue;
         const int64_t ix = row * g.ncols + col;
-        const double *pp = partials + (int64_t)sp.y * 4096 + (wave * 16 + j) * 64 + lane;
+        const double *pp = partials + (int64_t)sp.y * MKDE_DT_CELLS + k;
         double t = 0.0;
-        for (int k = 0; k < sp.z; ++k) t += pp[(int64_t)k * 4096];
+        for (int q = 0; q < sp.z; ++q) t += pp[(int64_t)q * MKDE_DT_CELLS];
         const bool valid = !mask || mask[ix] != g.nodata;
         out[ix] = valid ? (float)t : mask[ix];
     }
@@ -2231,14 +2257,14 @@ hipError_t bin_stats(hipStream_t s, int nbins, const int *bin_start, const PtRec
     return hipGetLastError();
 }
 
-// 64 x 64 blocks of a single tile or of a tiled raster (tile-major numbering)
+// Density blocks of a single tile or of a tiled raster (tile-major numbering)
 int64_t raster_blocks(const GridDesc &g, const Tiling &tl)
 {
     if (tl.tw > 0) {
         const int64_t nty = (g.nrows + tl.th - 1) / tl.th;
         return (int64_t)tl.ntx * nty * tl.bpt;
     }
-    return ((g.ncols + 63) / 64) * ((g.nrows + 63) / 64);
+    return ((g.ncols + MKDE_DT_BW - 1) / MKDE_DT_BW) * ((g.nrows + 63) / 64);
 }
 
 static void scan_u32(hipStream_t s, int n, const unsigned *in, unsigned *out, unsigned *chunks, unsigned *total)
@@ -2253,7 +2279,7 @@ static void scan_u32(hipStream_t s, int n, const unsigned *in, unsigned *out, un
 hipError_t density_plan(hipStream_t s, const GridDesc &g, const float *mask, const PointIndex &idx, const Tiling &tl,
                         const DensityWork &W)
 {
-    const int nbx = (int)((g.ncols + 63) / 64);
+    const int nbx = (int)((g.ncols + MKDE_DT_BW - 1) / MKDE_DT_BW);
     const int nblk = (int)raster_blocks(g, tl);
     hipLaunchKernelGGL(k_block_cost, dim3((unsigned)nblk), dim3(256), 0, s, g, mask, idx, nbx, tl, W.cost);
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, (int64_t)2, 0.0f, (float *)W.total);
@@ -2294,11 +2320,11 @@ hipError_t density_tile(hipStream_t s, const GridDesc &g, const float *mask, flo
                         unsigned long long *stats, const int *order, const int4 *items, const unsigned *cost,
                         double *partials, int n_items, const Tiling &tl)
 {
-    const int nbx = (int)((g.ncols + 63) / 64);
+    const int nbx = (int)((g.ncols + MKDE_DT_BW - 1) / MKDE_DT_BW);
     TileArgs A{g, mask, out, idx, stats, order, items, cost, partials, nbx, tl};
     dim3 grid((unsigned)(order ? n_items : raster_blocks(g, tl)));
-    if (stats) hipLaunchKernelGGL(k_density_tile<true>, grid, dim3(256), 0, s, A);
-    else hipLaunchKernelGGL(k_density_tile<false>, grid, dim3(256), 0, s, A);
+    if (stats) hipLaunchKernelGGL(k_density_tile<true>, grid, dim3(DT_NT), 0, s, A);
+    else hipLaunchKernelGGL(k_density_tile<false>, grid, dim3(DT_NT), 0, s, A);
     return hipGetLastError();
 }
 
@@ -2306,7 +2332,7 @@ hipError_t density_merge(hipStream_t s, const GridDesc &g, const Tiling &tl, con
                          const int4 *splits, const double *partials, float *out)
 {
     if (n_split <= 0) return hipSuccess;
-    const int nbx = (int)((g.ncols + 63) / 64);
+    const int nbx = (int)((g.ncols + MKDE_DT_BW - 1) / MKDE_DT_BW);
     hipLaunchKernelGGL(k_density_merge, dim3((unsigned)n_split), dim3(256), 0, s, g, tl, nbx, mask, splits, partials,
                        out);
     return hipGetLastError();

@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "mkde.h"
+#include "mkde_device.h"
 
 extern "C" int mkde_internal_fail(int code, const char *msg);
 
@@ -120,10 +121,10 @@ extern "C" int64_t mkde_job_footprint(int64_t n, int64_t ncols, int64_t nrows, i
     // 2 * cost / 4096 parts per wave (cost <= n) and at most twice the split
     // target: 2 * 16384 (density at points, 512 B each) or 2 * 32768
     // (16-candidate pass, 8 KB each; MKDE_LOO_TARGET is not taken into account)
-    // FP64 partials of split 64x64 density blocks (32 KB each): at most twice
-    // the split target (2 * 4096) and 2 * cost / 2048 + 1 per block (cost <= n)
-    const double nblk = (double)((ncols + 63) / 64) * (double)((nrows + 63) / 64);
-    b += (int64_t)std::min(8192.0, 2.0 * (double)n * nblk / 2048.0 + nblk) * 32768;
+    // FP64 partials of split density blocks (8 B per cell): at most twice the
+    // split target (2 * 4096) and 2 * cost / 2048 + 1 per block (cost <= n)
+    const double nblk = (double)((ncols + MKDE_DT_BW - 1) / MKDE_DT_BW) * (double)((nrows + 63) / 64);
+    b += (int64_t)std::min(8192.0, 2.0 * (double)n * nblk / 2048.0 + nblk) * (8 * MKDE_DT_CELLS);
     const int64_t nw = (n + 63) / 64;
     const double parts = 2.0 * (double)nw * (double)n / 4096.0;
     b += (int64_t)std::min(parts, 32768.0) * 512;
