@@ -21,8 +21,10 @@
  *     separate contexts are independent.
  *   - Floating point follows the reference CPU path: single-precision
  *     coordinates, the cut-off predicate fl64(dx^2+dy^2) < 9*fl32(h*h) with
- *     dx = fl32(x1-x0) evaluated exactly, FP32 compensated sums for densities
- *     and edge-correction integrals, FP64 for log-likelihood sums.
+ *     dx = fl32(x1-x0) evaluated exactly; densities at points in FP32 with
+ *     compensated folds, the density raster in FP32 partial sums of at most
+ *     64 terms folded into FP64, edge-correction integrals from FP64 prefix
+ *     sums, FP64 for log-likelihood sums.
  */
 #ifndef MKDE_H
 #define MKDE_H
@@ -212,8 +214,9 @@ int mkde_density_raster(mkde_ctx *ctx, int64_t ncols, int64_t nrows, const doubl
 /* Diagnostic counters of the density kernel (no reference counterpart).
  * enable != 0 zeroes and enables them; counters (8 entries, may be NULL)
  * receives: [0] candidate points classified, [1] FULL and [2] PARTIAL
- * (point, 64x64 block) pairs, [3] FULL, [4] PARTIAL and [5] EMPTY (point,
- * 64x16 wave sub-block) pairs of PARTIAL points, [6] blocks without a valid
+ * (point, 64x64 block) pairs, [3] FULL (point, 32x32 wave quadrant) pairs
+ * (the quadrants of block-FULL points included), [4] PARTIAL and [5] EMPTY
+ * (point, quadrant) pairs of block-PARTIAL points, [6] blocks without a valid
  * cell; enable == 0 reads them and switches them off. */
 int mkde_density_stats(mkde_ctx *ctx, int enable, int64_t *counters);
 
