@@ -1320,10 +1320,12 @@ __global__ __launch_bounds__(256) void k_loo_multi_w(LooMultiArgs A)
     // source's d2f instead (d2f * (1 + 2^-20) < t: inside for sure;
     // d2f * (1 - 2^-20) >= t: outside for sure), which keeps the candidate
     // constants within the scalar registers
+    // Slots k >= K get an infinite threshold (their sums are never written
+    // out), so the candidate loop needs no bound test.
     float tf[LM_K], af[LM_K];
 #pragma unroll
     for (int k = 0; k < LM_K; ++k) {
-        tf[k] = uni(k < K ? (float)A.thr_k[k] : -1.0f);
+        tf[k] = uni(k < K ? (float)A.thr_k[k] : INFINITY);
         af[k] = uni(k < K ? (float)A.a_k[k] : 0.0f);
     }
     const double tmax = A.thr_k[K - 1];
@@ -1361,19 +1363,28 @@ __global__ __launch_bounds__(256) void k_loo_multi_w(LooMultiArgs A)
                     const float cf[LM_K] = {cf0.x, cf0.y, cf0.z, cf0.w, cf1.x, cf1.y, cf1.z, cf1.w,
                                             cf2.x, cf2.y, cf2.z, cf2.w, cf3.x, cf3.y, cf3.z, cf3.w};
                     const float dx = p.x - qx, dy = p.y - qy;
-                    const float d2f = __builtin_fmaf(dy, dy, dx * dx);
+                    // the query itself (leave-one-out) tests outside every
+                    // cut-off: its term is 0 for every candidate
+                    const float d2f = (int64_t)si == q ? INFINITY : __builtin_fmaf(dy, dy, dx * dx);
                     const float d2_in = d2f * (float)(1.0 + PD_BAND), d2_maybe = d2f * (float)(1.0 - PD_BAND);
-                    const bool self = (int64_t)si == q;
+                    // wave-uniform lower bound of d2_maybe: the source's gap to
+                    // the queries' bounding box (FP32 gaps round like the lanes'
+                    // differences, so no lane is below it); candidates are
+                    // visited from the largest down while it is below their
+                    // threshold, compared as the bit patterns of non-negative
+                    // floats on the scalar unit
+                    const float gx = fmaxf(fmaxf(mnx - p.x, p.x - mxx), 0.0f);
+                    const float gy = fmaxf(fmaxf(mny - p.y, p.y - mxy), 0.0f);
+                    const int lbb = __builtin_amdgcn_readfirstlane(
+                        __float_as_int(__builtin_fmaf(gy, gy, gx * gx) * (float)(1.0 - 2.0 * PD_BAND)));
 #pragma unroll
                     for (int k = LM_K - 1; k >= 0; --k) {
-                        if (k < K) {
-                            const bool maybe = d2_maybe < tf[k];
-                            if (!__any(maybe)) break;
-                            const bool in = d2_in < tf[k];
-                            band |= maybe != in;
-                            const float e = fast_exp2(d2f * af[k]);
-                            acc[k] = __builtin_fmaf((in && !self) ? e : 0.0f, cf[k], acc[k]);
-                        }
+                        if (!(lbb < __float_as_int(tf[k]))) break;
+                        const bool maybe = d2_maybe < tf[k];
+                        const bool in = d2_in < tf[k];
+                        band |= maybe != in;
+                        const float e = fast_exp2(d2f * af[k]);
+                        acc[k] = __builtin_fmaf(in ? e : 0.0f, cf[k], acc[k]);
                     }
                 }
                 if (__any(band)) {   // rare: exact FP64 cut-off for the chunk
