@@ -1292,7 +1292,23 @@ struct LooMultiArgs {
 // walk stops at the first one with no lane inside its (upper) cut-off.
 // FP32 screen; a 32-source chunk with any lane inside a 2^-20 band is redone
 // with the exact FP64 test.
-__global__ __launch_bounds__(256) void k_loo_multi_w(LooMultiArgs A)
+// Wave minimum of a non-negative float (or +inf), as its bit pattern in an
+// SGPR: rotations within each 16-lane DPP row, then the four row minima on
+// the scalar unit (the bit patterns of non-negative floats order like ints).
+__device__ __forceinline__ int wave_min_nonneg_bits(float v)
+{
+    int b = __float_as_int(v);
+    b = min(b, __builtin_amdgcn_mov_dpp(b, 0x128, 0xf, 0xf, false));   // row_ror:8
+    b = min(b, __builtin_amdgcn_mov_dpp(b, 0x124, 0xf, 0xf, false));   // row_ror:4
+    b = min(b, __builtin_amdgcn_mov_dpp(b, 0x122, 0xf, 0xf, false));   // row_ror:2
+    b = min(b, __builtin_amdgcn_mov_dpp(b, 0x121, 0xf, 0xf, false));   // row_ror:1
+    const int r0 = __builtin_amdgcn_readlane(b, 0), r1 = __builtin_amdgcn_readlane(b, 16);
+    const int r2 = __builtin_amdgcn_readlane(b, 32), r3 = __builtin_amdgcn_readlane(b, 48);
+    return min(min(r0, r1), min(r2, r3));
+}
+
+template <bool FULLK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_loo_multi_w(LooMultiArgs A)
 {
     const PointIndex &I = A.idx;
     const int lane = threadIdx.x & 63;
@@ -1320,12 +1336,10 @@ __global__ __launch_bounds__(256) void k_loo_multi_w(LooMultiArgs A)
     // source's d2f instead (d2f * (1 + 2^-20) < t: inside for sure;
     // d2f * (1 - 2^-20) >= t: outside for sure), which keeps the candidate
     // constants within the scalar registers
-    // Slots k >= K get an infinite threshold (their sums are never written
-    // out), so the candidate loop needs no bound test.
     float tf[LM_K], af[LM_K];
 #pragma unroll
     for (int k = 0; k < LM_K; ++k) {
-        tf[k] = uni(k < K ? (float)A.thr_k[k] : INFINITY);
+        tf[k] = uni(k < K ? (float)A.thr_k[k] : -1.0f);
         af[k] = uni(k < K ? (float)A.a_k[k] : 0.0f);
     }
     const double tmax = A.thr_k[K - 1];
@@ -1363,28 +1377,43 @@ __global__ __launch_bounds__(256) void k_loo_multi_w(LooMultiArgs A)
                     const float cf[LM_K] = {cf0.x, cf0.y, cf0.z, cf0.w, cf1.x, cf1.y, cf1.z, cf1.w,
                                             cf2.x, cf2.y, cf2.z, cf2.w, cf3.x, cf3.y, cf3.z, cf3.w};
                     const float dx = p.x - qx, dy = p.y - qy;
-                    // the query itself (leave-one-out) tests outside every
-                    // cut-off: its term is 0 for every candidate
-                    const float d2f = (int64_t)si == q ? INFINITY : __builtin_fmaf(dy, dy, dx * dx);
-                    const float d2_in = d2f * (float)(1.0 + PD_BAND), d2_maybe = d2f * (float)(1.0 - PD_BAND);
-                    // wave-uniform lower bound of d2_maybe: the source's gap to
-                    // the queries' bounding box (FP32 gaps round like the lanes'
-                    // differences, so no lane is below it); candidates are
-                    // visited from the largest down while it is below their
-                    // threshold, compared as the bit patterns of non-negative
-                    // floats on the scalar unit
-                    const float gx = fmaxf(fmaxf(mnx - p.x, p.x - mxx), 0.0f);
-                    const float gy = fmaxf(fmaxf(mny - p.y, p.y - mxy), 0.0f);
-                    const int lbb = __builtin_amdgcn_readfirstlane(
-                        __float_as_int(__builtin_fmaf(gy, gy, gx * gx) * (float)(1.0 - 2.0 * PD_BAND)));
+                    const bool self = (int64_t)si == q;
+                    if (FULLK) {
+                        // every slot holds a candidate.  The query itself
+                        // (leave-one-out) tests outside every cut-off: its
+                        // term is 0 for every candidate.  The wave minimum of
+                        // d2_maybe, as a uniform bit pattern, ends the walk:
+                        // candidates are visited from the largest down while
+                        // some lane may be inside (the same set a per-candidate
+                        // vote gives), compared on the scalar unit as the bit
+                        // patterns of non-negative floats
+                        const float d2f = self ? INFINITY : __builtin_fmaf(dy, dy, dx * dx);
+                        const float d2_in = d2f * (float)(1.0 + PD_BAND), d2_maybe = d2f * (float)(1.0 - PD_BAND);
+                        const int lbb = wave_min_nonneg_bits(d2_maybe);
 #pragma unroll
-                    for (int k = LM_K - 1; k >= 0; --k) {
-                        if (!(lbb < __float_as_int(tf[k]))) break;
-                        const bool maybe = d2_maybe < tf[k];
-                        const bool in = d2_in < tf[k];
-                        band |= maybe != in;
-                        const float e = fast_exp2(d2f * af[k]);
-                        acc[k] = __builtin_fmaf(in ? e : 0.0f, cf[k], acc[k]);
+                        for (int k = LM_K - 1; k >= 0; --k) {
+                            if (!(lbb < __float_as_int(tf[k]))) break;
+                            const bool maybe = d2_maybe < tf[k];
+                            const bool in = d2_in < tf[k];
+                            band |= maybe != in;
+                            const float e = fast_exp2(d2f * af[k]);
+                            acc[k] = __builtin_fmaf(in ? e : 0.0f, cf[k], acc[k]);
+                        }
+                    } else {
+                        // K < 16: a wave-wide vote per candidate ends the walk
+                        const float d2f = __builtin_fmaf(dy, dy, dx * dx);
+                        const float d2_in = d2f * (float)(1.0 + PD_BAND), d2_maybe = d2f * (float)(1.0 - PD_BAND);
+#pragma unroll
+                        for (int k = LM_K - 1; k >= 0; --k) {
+                            if (k < K) {
+                                const bool maybe = d2_maybe < tf[k];
+                                if (!__any(maybe)) break;
+                                const bool in = d2_in < tf[k];
+                                band |= maybe != in;
+                                const float e = fast_exp2(d2f * af[k]);
+                                acc[k] = __builtin_fmaf((in && !self) ? e : 0.0f, cf[k], acc[k]);
+                            }
+                        }
                     }
                 }
                 if (__any(band)) {   // rare: exact FP64 cut-off for the chunk
@@ -2421,7 +2450,10 @@ hipError_t loo_multi(hipStream_t s, const PointIndex &idx, const float *coefk, c
     LooMultiArgs A{idx, coefk, a_k, thr_k, K, den1, order, items, cost, partials, n_items};
     if (idx.n <= 0) return hipSuccess;
     const int64_t units = order ? n_items : (idx.n + 63) / 64;
-    hipLaunchKernelGGL(k_loo_multi_w, dim3((unsigned)((units + 3) / 4)), dim3(256), 0, s, A);
+    // full 16-candidate passes (every pass of a bandwidth search) take the
+    // loop with one uniform bound per source
+    if (K == MKDE_LM_K) hipLaunchKernelGGL(k_loo_multi_w<true>, dim3((unsigned)((units + 3) / 4)), dim3(256), 0, s, A);
+    else hipLaunchKernelGGL(k_loo_multi_w<false>, dim3((unsigned)((units + 3) / 4)), dim3(256), 0, s, A);
     return hipGetLastError();
 }
 
