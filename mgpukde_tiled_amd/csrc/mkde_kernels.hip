@@ -628,9 +628,20 @@ __global__ __launch_bounds__(DT_NT) __attribute__((amdgpu_waves_per_eu(4, 4))) v
             for (int j = 0; j < 16; ++j) c[j] = acc[j];
             const int k = lane >> 5, li = lane & 31;
             const float *eyp = s_ey + DT_WY0(wave) + li, *exp = s_ex + DT_WX0(wave) + li;
-            // block-FULL points four per step (the second pair's operands
-            // arrive while the first pair's MFMA runs), then the rest in pairs
+            // block-FULL points eight, then four per step (the later pairs'
+            // operands arrive while the first pair's MFMA runs), then the
+            // rest in pairs
             int p = 0;
+            for (; p + 8 <= nfull; p += 8) {
+                const float ey0 = eyp[(p + k) * DT_EYP], ex0 = exp[(p + k) * MKDE_DT_BW];
+                const float ey1 = eyp[(p + 2 + k) * DT_EYP], ex1 = exp[(p + 2 + k) * MKDE_DT_BW];
+                const float ey2 = eyp[(p + 4 + k) * DT_EYP], ex2 = exp[(p + 4 + k) * MKDE_DT_BW];
+                const float ey3 = eyp[(p + 6 + k) * DT_EYP], ex3 = exp[(p + 6 + k) * MKDE_DT_BW];
+                c = __builtin_amdgcn_mfma_f32_32x32x2f32(ey0, ex0, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_32x32x2f32(ey1, ex1, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_32x32x2f32(ey2, ex2, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_32x32x2f32(ey3, ex3, c, 0, 0, 0);
+            }
             for (; p + 4 <= nfull; p += 4) {
                 const float ey0 = eyp[(p + k) * DT_EYP], ex0 = exp[(p + k) * MKDE_DT_BW];
                 const float ey1 = eyp[(p + 2 + k) * DT_EYP], ex1 = exp[(p + 2 + k) * MKDE_DT_BW];
