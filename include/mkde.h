@@ -107,8 +107,9 @@ int mkde_mark_boundary(mkde_ctx *ctx, uint8_t *flags, int64_t *n_boundary);
 int mkde_dist_to_boundary(mkde_ctx *ctx, double *dist);
 int mkde_set_dist_to_boundary(mkde_ctx *ctx, const double *dist);
 
-/* Bandwidths: hs[] of the reference (tiled:1-, AllocateBandwidths 2878,
- * CopyToDeviceBandwidths; Hoption -1 reads them from the CSV, tiled:413-420). */
+/* Bandwidths: hs[] of the reference (AllocateBandwidths tiled:2878-2885,
+ * AllocateDeviceBandwidths tiled:2887-2901, CopyToDeviceBandwidths
+ * tiled:2903; Hoption -1 reads them from the CSV, tiled:413-420). */
 int mkde_set_bandwidth_fixed(mkde_ctx *ctx, float h);
 int mkde_set_bandwidths(mkde_ctx *ctx, const float *h);
 int mkde_get_bandwidths(mkde_ctx *ctx, float *h);
@@ -191,7 +192,7 @@ int mkde_tile_download(mkde_ctx *ctx, float *out);
 
 /* Every tile of a GeoTIFF-style raster in one pass (replaces the tile loop of
  * main(), tiled:1235-1309 with GeoTIFFReader::GetRasterBand_NextTile,
- * GeoTIFF.cpp:150-197, and AsciiRasterFromGeoTIFFTile, tiled:2193-2262).
+ * GeoTIFF.cpp:333-476, and AsciiRasterFromGeoTIFFTile, tiled:2193-2262).
  * The raster has ncols x nrows cells and geotransform gt[6] (GDAL order);
  * tiles are tile_cols x tile_rows cells from the top-left, edge tiles
  * smaller.  Each tile's cells use that tile's origin exactly as the
@@ -306,7 +307,9 @@ int64_t mkde_lzw_decode(const uint8_t *in, int64_t n, uint8_t *out, int64_t cap)
  * A job server on one device (no reference counterpart: the reference runs
  * one job per process).  `slots` worker threads each own a context (its own
  * stream; device buffers grow to the largest job seen and are reused).  Jobs
- * start in submission order when a slot is free and HBM admission allows it:
+ * start in submission order when a slot is free and HBM admission allows it
+ * (a later job that fits may start ahead of a head that does not, at most 16
+ * times per head):
  * the device bytes held by all slots, plus the footprint bound
  * (mkde_job_footprint) of every running job, plus the new job's bound must
  * not exceed hbm_budget (<= 0: 80% of the device's free memory at creation).

@@ -144,6 +144,9 @@ class ServerInfo(ctypes.Structure):
 
 
 MKDE_H_SEARCH = 3
+MKDE_E_ARG = 1
+MKDE_E_STATE = 4
+MKDE_E_RANGE = 6
 MKDE_E_NOMEM = 3
 MKDE_E_PENDING = 7
 MKDE_E_CANCELLED = 8
@@ -247,6 +250,19 @@ def _p(a, t):
     return None if a is None else a.ctypes.data_as(t)
 
 
+def _vec(a, n, what, dtype=np.float32):
+    """1-D contiguous copy of a with exactly n entries (the native side reads n)."""
+    v = np.ascontiguousarray(a, dtype=dtype).reshape(-1)
+    if v.size != n:
+        raise ValueError("%s has %d entries, expected %d" % (what, v.size, n))
+    return v
+
+
+def _mask(mask, g, what="mask"):
+    """None, or the grid's ncols*nrows mask values as a contiguous float32 vector."""
+    return None if mask is None else _vec(mask, int(g.ncols) * int(g.nrows), what)
+
+
 class Context:
     """A device context (one HIP device, one stream) with resident state."""
 
@@ -280,8 +296,9 @@ class Context:
     # -- inputs ---------------------------------------------------------
     def points_upload(self, x, y, w=None):
         """CopyToDeviceSamplePoints + index build (replaces BuildGPUKDtree)."""
-        x, y = _f32(x), _f32(y)
-        w = None if w is None else _f32(w)
+        x = _f32(x).reshape(-1)
+        y = _vec(y, x.size, "y")
+        w = None if w is None else _vec(w, x.size, "w")
         self._x, self._y, self._w = x, y, w
         _check(_lib.mkde_points_upload(self._h, len(x), _p(x, _FP), _p(y, _FP), _p(w, _FP)))
         self.n = len(x)
@@ -313,14 +330,14 @@ class Context:
         return d
 
     def set_dist_to_boundary(self, d):
-        d = np.ascontiguousarray(d, np.float64)
+        d = _vec(d, self.n, "distances", np.float64)
         _check(_lib.mkde_set_dist_to_boundary(self._h, _p(d, _DP)))
 
     def set_bandwidth(self, h):
         _check(_lib.mkde_set_bandwidth_fixed(self._h, float(h)))
 
     def set_bandwidths(self, hs):
-        hs = _f32(hs)
+        hs = _vec(hs, self.n, "bandwidths")
         _check(_lib.mkde_set_bandwidths(self._h, _p(hs, _FP)))
 
     def get_bandwidths(self):
@@ -336,7 +353,7 @@ class Context:
         return ew, ne.value
 
     def set_ec_weights(self, ew=None):
-        ew = None if ew is None else _f32(ew)
+        ew = None if ew is None else _vec(ew, self.n, "EC weights")
         _check(_lib.mkde_set_ec_weights(self._h, _p(ew, _FP)))
 
     def get_ec_weights(self):
@@ -400,20 +417,20 @@ class Context:
 
     def KernelDensityEstimation_tile(self, tile, mask=None):
         """Density raster of one tile (nodata cells keep the mask value)."""
-        m = None if mask is None else _f32(mask).reshape(-1)
+        m = _mask(mask, tile)
         out = np.empty(tile.ncols * tile.nrows, np.float32)
         _check(_lib.mkde_density_tile(self._h, ctypes.byref(tile), _p(m, _FP), _p(out, _FP)))
         return out.reshape(tile.nrows, tile.ncols)
 
     def count_density_pairs(self, tile, mask=None):
-        m = None if mask is None else _f32(mask).reshape(-1)
+        m = _mask(mask, tile)
         n = ctypes.c_int64()
         _check(_lib.mkde_count_density_pairs(self._h, ctypes.byref(tile), _p(m, _FP), ctypes.byref(n)))
         return n.value
 
     # -- device-resident tile API (throughput measurement) ----------------
     def tile_upload(self, tile, mask=None):
-        m = None if mask is None else _f32(mask).reshape(-1)
+        m = _mask(mask, tile)
         self._tile = tile
         _check(_lib.mkde_tile_upload(self._h, ctypes.byref(tile), _p(m, _FP)))
 
@@ -443,9 +460,8 @@ class Context:
     def raster_upload(self, ncols, nrows, gt, nodata=-9999.0, tile_cols=4096, tile_rows=4096, mask=None):
         """Make a GeoTIFF-style raster of tiles the current tile (each tile
         with the origin AsciiRasterFromGeoTIFFTile gives it)."""
-        m = None if mask is None else _f32(mask).reshape(-1)
-        gtv = np.ascontiguousarray(gt, np.float64)
-        assert gtv.size == 6
+        m = None if mask is None else _vec(mask, int(ncols) * int(nrows), "mask")
+        gtv = _vec(gt, 6, "geotransform", np.float64)
         self._tile = grid(int(ncols), int(nrows), 0.0, 0.0, 1.0, nodata)
         _check(_lib.mkde_raster_upload(self._h, int(ncols), int(nrows), _p(gtv, _DP), float(nodata), int(tile_cols),
                                        int(tile_rows), _p(m, _FP)))
@@ -455,9 +471,8 @@ class Context:
         """Density of every tile of a raster in one pass (the reference's
         GeoTIFF tile loop); returns the nrows x ncols raster, or None with
         keep_on_device (then tile_download() copies it out)."""
-        m = None if mask is None else _f32(mask).reshape(-1)
-        gtv = np.ascontiguousarray(gt, np.float64)
-        assert gtv.size == 6
+        m = None if mask is None else _vec(mask, int(ncols) * int(nrows), "mask")
+        gtv = _vec(gt, 6, "geotransform", np.float64)
         self._tile = grid(int(ncols), int(nrows), 0.0, 0.0, 1.0, nodata)
         out = None if keep_on_device else np.empty(int(ncols) * int(nrows), np.float32)
         _check(_lib.mkde_density_raster(self._h, int(ncols), int(nrows), _p(gtv, _DP), float(nodata),
@@ -514,13 +529,15 @@ class Server:
                alpha=-0.5, n_candidates=16, cand_lo=0.1, cand_hi=10.0, one_candidate_per_pass=False):
         """Queue one job; returns its ticket.  The density lands in `out`
         (allocated here when None) once wait() reports it finished."""
-        x, y = _f32(x), _f32(y)
-        w = None if w is None else _f32(w)
-        h = None if h is None else _f32(h)
-        m = None if mask is None else _f32(mask).reshape(-1)
+        x = _f32(x).reshape(-1)
+        y = _vec(y, x.size, "y")
+        w = None if w is None else _vec(w, x.size, "w")
+        h = None if h is None else _vec(h, x.size, "h")
+        m = _mask(mask, g)
         if out is None:
             out = np.empty((g.nrows, g.ncols), np.float32)
-        assert out.dtype == np.float32 and out.flags.c_contiguous and out.size == g.ncols * g.nrows
+        if not (out.dtype == np.float32 and out.flags.c_contiguous and out.size == g.ncols * g.nrows):
+            raise ValueError("out must be a contiguous float32 array of ncols*nrows values")
         j = ServeJob(len(x), _p(x, _FP), _p(y, _FP), _p(w, _FP), _p(h, _FP), g, _p(m, _FP), int(hoption),
                      1 if edge_correction else 0, 1 if optimise else 0, float(alpha), int(n_candidates),
                      float(cand_lo), float(cand_hi), _p(out, _FP), 1 if one_candidate_per_pass else 0)
@@ -620,7 +637,8 @@ def csv_read_points(path, with_h=False, sample_weights=False):
 
 def csv_write_points(path, x, y, h, w):
     load_library()
-    x, y, h, w = _f32(x), _f32(y), _f32(h), _f32(w)
+    x = _f32(x).reshape(-1)
+    y, h, w = _vec(y, x.size, "y"), _vec(h, x.size, "h"), _vec(w, x.size, "w")
     _check(_lib.mkde_csv_write_points(_b(path), len(x), _p(x, _FP), _p(y, _FP), _p(h, _FP), _p(w, _FP)))
 
 

@@ -16,6 +16,7 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmkde.so")
 CLI = os.path.join(HERE, "mgpukde_tiled")
+RATES = os.path.join(HERE, "mkde_rates")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -55,6 +56,10 @@ def build(verbose=True):
     if os.path.exists(cli_src):
         _run([HIPCC] + COMMON + HOST_FLAGS + ["-x", "c++", cli_src, "-o", CLI, "-L" + HERE, "-lmkde",
                                  "-Wl,-rpath,$ORIGIN"])
+    # instruction-rate probe (exp / FMA / MFMA throughput) used by bench.py's roofline
+    rates_src = os.path.join(CSRC, "mkde_rates.hip")
+    if os.path.exists(rates_src):
+        _run([HIPCC] + DEVICE_FLAGS + COMMON + [rates_src, "-o", RATES])
     return LIB
 
 

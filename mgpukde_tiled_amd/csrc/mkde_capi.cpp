@@ -149,6 +149,9 @@ struct mkde_ctx {
     std::atomic<int64_t> dev_bytes{0};   // device memory held by the buffers below
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // events of a job's phase timers, created once and reused by every job
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_next = 0;
     float last_density_ms = 0.0f, last_ec_ms = 0.0f, last_points_ms = 0.0f;
 
     // points, original order (host copies kept for the rule of thumb)
@@ -287,6 +290,7 @@ extern "C" int mkde_ctx_destroy(mkde_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     t_acct = nullptr;
@@ -309,6 +313,8 @@ extern "C" int mkde_ctx_release(mkde_ctx *c)
     const int dev = c->device;
     hipStream_t s = c->stream;
     hipEvent_t e0 = c->ev0, e1 = c->ev1;
+    std::vector<hipEvent_t> pool;
+    pool.swap(c->ev_pool);
     const bool order_on = c->dt_order_on;
     c->~mkde_ctx();
     new (c) mkde_ctx();
@@ -316,6 +322,7 @@ extern "C" int mkde_ctx_release(mkde_ctx *c)
     c->stream = s;
     c->ev0 = e0;
     c->ev1 = e1;
+    c->ev_pool.swap(pool);
     c->dt_order_on = order_on;
     t_acct = &c->dev_bytes;
     return MKDE_OK;
@@ -351,19 +358,26 @@ static int copy_d2h(mkde_ctx *c, void *dst, const void *src, size_t bytes)
 // 4M bins; a stable counting sort gives the bin order (ties keep the caller's
 // order, so every downstream summation order is deterministic).
 
-static std::vector<int> build_bins(mkde_ctx *c)
+struct BinBuild {
+    BinGrid bg{};
+    int nbins = 0;
+    std::vector<int> perm;    // bin-order position -> caller's index
+    std::vector<int> start;   // nbins + 1
+};
+
+static void build_bins(const std::vector<float> &hx, const std::vector<float> &hy, int pts_per_bin, BinBuild &B)
 {
-    const int64_t n = c->n;
+    const int64_t n = (int64_t)hx.size();
     float x0 = FLT_MAX, x1 = -FLT_MAX, y0 = FLT_MAX, y1 = -FLT_MAX;
     for (int64_t i = 0; i < n; ++i) {
-        x0 = std::min(x0, c->hx[i]);
-        x1 = std::max(x1, c->hx[i]);
-        y0 = std::min(y0, c->hy[i]);
-        y1 = std::max(y1, c->hy[i]);
+        x0 = std::min(x0, hx[i]);
+        x1 = std::max(x1, hx[i]);
+        y0 = std::min(y0, hy[i]);
+        y1 = std::max(y1, hy[i]);
     }
     if (n == 0) { x0 = y0 = 0.0f; x1 = y1 = 1.0f; }
     double wx = std::max((double)x1 - x0, 1e-6), wy = std::max((double)y1 - y0, 1e-6);
-    double target = std::max(1.0, (double)n / (double)c->pts_per_bin);
+    double target = std::max(1.0, (double)n / (double)pts_per_bin);
     double size = std::sqrt(wx * wy / target);
     size = std::max(size, std::max(wx, wy) / 4096.0);
     int nbx, nby;
@@ -373,22 +387,34 @@ static std::vector<int> build_bins(mkde_ctx *c)
         if ((int64_t)nbx * nby <= (4 << 20)) break;
         size *= 1.25;
     }
-    c->bg = BinGrid{(double)x0, (double)y0, size, nbx, nby};
-    c->nbins = nbx * nby;
-    std::vector<int> key(n), start((size_t)c->nbins + 1, 0);
+    B.bg = BinGrid{(double)x0, (double)y0, size, nbx, nby};
+    B.nbins = nbx * nby;
+    std::vector<int> key(n);
+    B.start.assign((size_t)B.nbins + 1, 0);
     for (int64_t i = 0; i < n; ++i) {
-        int bx = (int)std::floor(((double)c->hx[i] - x0) / size);
-        int by = (int)std::floor(((double)c->hy[i] - y0) / size);
+        int bx = (int)std::floor(((double)hx[i] - x0) / size);
+        int by = (int)std::floor(((double)hy[i] - y0) / size);
         bx = std::min(std::max(bx, 0), nbx - 1);
         by = std::min(std::max(by, 0), nby - 1);
         key[i] = by * nbx + bx;
-        start[key[i] + 1]++;
+        B.start[key[i] + 1]++;
     }
-    for (int b = 0; b < c->nbins; ++b) start[b + 1] += start[b];
-    c->hperm.assign(n, 0);
-    std::vector<int> pos(start.begin(), start.end() - 1);
-    for (int64_t i = 0; i < n; ++i) c->hperm[pos[key[i]]++] = (int)i;
-    return start;
+    for (int b = 0; b < B.nbins; ++b) B.start[b + 1] += B.start[b];
+    B.perm.assign(n, 0);
+    std::vector<int> pos(B.start.begin(), B.start.end() - 1);
+    for (int64_t i = 0; i < n; ++i) B.perm[pos[key[i]]++] = (int)i;
+}
+
+// Drop every per-point state of the context (a failed upload leaves it
+// without points, so later calls report MKDE_E_STATE instead of using
+// buffers of another size).
+static void forget_points(mkde_ctx *c)
+{
+    c->n = 0;
+    c->h_set = false;
+    c->h_fixed = 0.0f;
+    c->dist_set = false;
+    c->pts_dirty = true;
 }
 
 extern "C" int mkde_points_upload(mkde_ctx *c, int64_t n, const float *x, const float *y, const float *w)
@@ -396,54 +422,66 @@ extern "C" int mkde_points_upload(mkde_ctx *c, int64_t n, const float *x, const 
     if (!c) return fail(MKDE_E_ARG, "null context");
     if (n < 0 || (n > 0 && (!x || !y))) return fail(MKDE_E_ARG, "mkde_points_upload: bad arguments");
     if (n > (int64_t)1 << 30) return fail(MKDE_E_RANGE, "too many points (%lld > 2^30)", (long long)n);
+    // validate before anything of the context changes: on this error the
+    // previous points stay usable
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(x[i]) || !std::isfinite(y[i]))
+            return fail(MKDE_E_ARG, "point %lld has a non-finite coordinate", (long long)i);
     CHK(set_device(c));
+    // build everything on the host in locals, then resize and fill the device
+    // buffers; the context takes the new points only when all of it succeeded
+    std::vector<float> hx, hy, hw, sx, sy;
+    BinBuild B;
     try {
-        c->n = n;
-        c->hx.assign(x, x + n);
-        c->hy.assign(y, y + n);
-        if (w) c->hw.assign(w, w + n);
-        else c->hw.assign(n, 1.0f);
-        for (int64_t i = 0; i < n; ++i)
-            if (!std::isfinite(c->hx[i]) || !std::isfinite(c->hy[i]))
-                return fail(MKDE_E_ARG, "point %lld has a non-finite coordinate", (long long)i);
-        std::vector<int> bs = build_bins(c);
-        std::vector<float> sx(n), sy(n);
+        hx.assign(x, x + n);
+        hy.assign(y, y + n);
+        if (w) hw.assign(w, w + n);
+        else hw.assign(n, 1.0f);
+        build_bins(hx, hy, c->pts_per_bin, B);
+        sx.resize(n);
+        sy.resize(n);
         for (int64_t i = 0; i < n; ++i) {
-            sx[i] = c->hx[c->hperm[i]];
-            sy[i] = c->hy[c->hperm[i]];
+            sx[i] = hx[B.perm[i]];
+            sy[i] = hy[B.perm[i]];
         }
-        CHK(c->x.ensure(n));
-        CHK(c->y.ensure(n));
-        CHK(c->w.ensure(n));
-        CHK(c->xs.ensure(n));
-        CHK(c->ys.ensure(n));
-        CHK(c->perm.ensure(n));
-        CHK(c->bin_start.ensure(c->nbins + 1));
-        CHK(c->pts.ensure(n));
-        CHK(c->bin_box.ensure(c->nbins));
-        CHK(c->bin_thr.ensure(c->nbins));
-        CHK(c->reach.ensure(1));
-        CHK(c->h.ensure(n));
-        CHK(c->ew.ensure(n));
-        CHK(c->dist.ensure(n));
-        CHK(c->den.ensure(n));
-        CHK(copy_h2d(c, c->x.p, c->hx.data(), n * sizeof(float)));
-        CHK(copy_h2d(c, c->y.p, c->hy.data(), n * sizeof(float)));
-        CHK(copy_h2d(c, c->w.p, c->hw.data(), n * sizeof(float)));
-        CHK(copy_h2d(c, c->xs.p, sx.data(), n * sizeof(float)));
-        CHK(copy_h2d(c, c->ys.p, sy.data(), n * sizeof(float)));
-        CHK(copy_h2d(c, c->perm.p, c->hperm.data(), n * sizeof(int)));
-        CHK(copy_h2d(c, c->bin_start.p, bs.data(), bs.size() * sizeof(int)));
     } catch (const std::bad_alloc &) {
         return fail(MKDE_E_NOMEM, "out of host memory");
     }
+    forget_points(c);   // from here on a failure leaves the context without points
+    HIPCHK(hipStreamSynchronize(c->stream));   // no queued work still reads the old buffers
+    CHK(c->x.ensure(n));
+    CHK(c->y.ensure(n));
+    CHK(c->w.ensure(n));
+    CHK(c->xs.ensure(n));
+    CHK(c->ys.ensure(n));
+    CHK(c->perm.ensure(n));
+    CHK(c->bin_start.ensure(B.nbins + 1));
+    CHK(c->pts.ensure(n));
+    CHK(c->bin_box.ensure(B.nbins));
+    CHK(c->bin_thr.ensure(B.nbins));
+    CHK(c->reach.ensure(1));
+    CHK(c->h.ensure(n));
+    CHK(c->ew.ensure(n));
+    CHK(c->dist.ensure(n));
+    CHK(c->den.ensure(n));
+    CHK(copy_h2d(c, c->x.p, hx.data(), n * sizeof(float)));
+    CHK(copy_h2d(c, c->y.p, hy.data(), n * sizeof(float)));
+    CHK(copy_h2d(c, c->w.p, hw.data(), n * sizeof(float)));
+    CHK(copy_h2d(c, c->xs.p, sx.data(), n * sizeof(float)));
+    CHK(copy_h2d(c, c->ys.p, sy.data(), n * sizeof(float)));
+    CHK(copy_h2d(c, c->perm.p, B.perm.data(), n * sizeof(int)));
+    CHK(copy_h2d(c, c->bin_start.p, B.start.data(), B.start.size() * sizeof(int)));
     HIPCHK(mkde_dev::fill_f32(c->stream, n, 1.0f, c->ew.p));
     HIPCHK(mkde_dev::fill_f32(c->stream, n, 0.0f, c->h.p));
-    c->h_set = false;
-    c->h_fixed = 0.0f;
-    c->dist_set = false;
-    c->pts_dirty = true;
     HIPCHK(hipStreamSynchronize(c->stream));
+    // commit
+    c->hx.swap(hx);
+    c->hy.swap(hy);
+    c->hw.swap(hw);
+    c->hperm.swap(B.perm);
+    c->bg = B.bg;
+    c->nbins = B.nbins;
+    c->n = n;
     return MKDE_OK;
 }
 
@@ -573,33 +611,38 @@ extern "C" int mkde_mask_upload(mkde_ctx *c, const mkde_grid *g, const float *va
     if (!values) return fail(MKDE_E_ARG, "mask values are NULL");
     CHK(set_device(c));
     const int64_t nc = g->ncols * g->nrows;
+    RunsHost R;
     try {
-        RunsHost R;
         build_runs(values, g->ncols, g->nrows, g->nodata, R);
-        c->mg = to_desc(g);
-        CHK(c->mask.ensure(nc));
-        CHK(c->valid.ensure(nc));
-        CHK(c->bflags.ensure(nc));
-        CHK(c->run_ptr.ensure(R.ptr.size()));
-        CHK(c->run_se.ensure(R.se.size()));
-        CHK(c->bcount.ensure(1));
-        CHK(copy_h2d(c, c->mask.p, values, nc * sizeof(float)));
-        CHK(copy_h2d(c, c->run_ptr.p, R.ptr.data(), R.ptr.size() * sizeof(int64_t)));
-        CHK(copy_h2d(c, c->run_se.p, R.se.data(), R.se.size() * sizeof(int2)));
     } catch (const std::bad_alloc &) {
         return fail(MKDE_E_NOMEM, "out of host memory");
     }
-    HIPCHK(mkde_dev::valid_cells(c->stream, nc, c->mask.p, g->nodata, c->valid.p));
-    CHK(c->colx.ensure(g->ncols));
-    CHK(c->rowy.ensure(g->nrows));
-    HIPCHK(mkde_dev::centres(c->stream, c->mg, c->colx.p, c->rowy.p));
-    c->mask_set = true;
+    // from here on a failure leaves the context without a mask (and without a
+    // density tile that was this mask), never with buffers of another grid
+    c->mask_set = false;
     c->bflags_ok = false;
     c->nboundary = -1;
     c->bt_ok = false;
     c->dist_set = false;
     if (c->tile_is_ec_mask) c->tile_set = c->tile_is_ec_mask = false;
+    HIPCHK(hipStreamSynchronize(c->stream));   // no queued work still reads the old buffers
+    CHK(c->mask.ensure(nc));
+    CHK(c->valid.ensure(nc));
+    CHK(c->bflags.ensure(nc));
+    CHK(c->run_ptr.ensure(R.ptr.size()));
+    CHK(c->run_se.ensure(R.se.size()));
+    CHK(c->bcount.ensure(1));
+    CHK(c->colx.ensure(g->ncols));
+    CHK(c->rowy.ensure(g->nrows));
+    CHK(copy_h2d(c, c->mask.p, values, nc * sizeof(float)));
+    CHK(copy_h2d(c, c->run_ptr.p, R.ptr.data(), R.ptr.size() * sizeof(int64_t)));
+    CHK(copy_h2d(c, c->run_se.p, R.se.data(), R.se.size() * sizeof(int2)));
+    const GridDesc gd = to_desc(g);
+    HIPCHK(mkde_dev::valid_cells(c->stream, nc, c->mask.p, g->nodata, c->valid.p));
+    HIPCHK(mkde_dev::centres(c->stream, gd, c->colx.p, c->rowy.p));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->mg = gd;
+    c->mask_set = true;
     return MKDE_OK;
 }
 
@@ -953,6 +996,30 @@ static int sum_logs(mkde_ctx *c, const float *den, int K, int clamp, double *out
 
 // --------------------------------------------------------------- LOO ----
 
+// Restores the context's bandwidth state when a likelihood scan returns, on
+// success and on every error path: the fixed bandwidth and its flag always,
+// and the bandwidth and EC-weight arrays from hsave / ewsave when the scan
+// overwrites them (the adaptive scan).
+struct BandwidthGuard {
+    mkde_ctx *c;
+    float hf;
+    bool hset;
+    bool arrays = false;
+    explicit BandwidthGuard(mkde_ctx *ctx) : c(ctx), hf(ctx->h_fixed), hset(ctx->h_set) {}
+    ~BandwidthGuard()
+    {
+        if (arrays) {
+            const size_t bytes = (size_t)c->n * sizeof(float);
+            (void)hipMemcpyAsync(c->h.p, c->hsave.p, bytes, hipMemcpyDeviceToDevice, c->stream);
+            (void)hipMemcpyAsync(c->ew.p, c->ewsave.p, bytes, hipMemcpyDeviceToDevice, c->stream);
+        }
+        (void)hipStreamSynchronize(c->stream);
+        c->h_fixed = hf;
+        c->h_set = hset;
+        c->pts_dirty = true;
+    }
+};
+
 extern "C" int mkde_loo_loglik(mkde_ctx *c, int K, const float *h_cand, int ec_on, double *logL)
 {
     CHK(need_points(c));
@@ -968,11 +1035,9 @@ extern "C" int mkde_loo_loglik(mkde_ctx *c, int K, const float *h_cand, int ec_o
     // positions for the index: prep with the largest candidate so that the bin
     // thresholds cover every candidate
     float hmax = h_cand[ord[K - 1]];
-    const float saved_hf = c->h_fixed;
-    const bool saved_hset = c->h_set;
-    DevBuf<float> &hsave = c->hsave;
-    CHK(hsave.ensure(n));
-    HIPCHK(hipMemcpyAsync(hsave.p, c->h.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    // the scan sets the fixed bandwidth to hmax for the index (the bandwidth
+    // array itself is never written); the guard restores the caller's state
+    BandwidthGuard guard(c);
     c->h_fixed = hmax;
     c->pts_dirty = true;
     CHK(prep(c));
@@ -981,27 +1046,32 @@ extern "C" int mkde_loo_loglik(mkde_ctx *c, int K, const float *h_cand, int ec_o
     CHK(c->coefk.ensure((size_t)MKDE_LM_K * n));   // [n][MKDE_LM_K]: a point's candidate slots contiguous
     CHK(c->den1k.ensure((size_t)G * n));
     CHK(c->ewk.ensure(n));
-    CHK(c->ak.ensure(G));
-    CHK(c->tk.ensure(G));
+    CHK(c->ak.ensure(MKDE_LM_K));
+    CHK(c->tk.ensure(MKDE_LM_K));
     CHK(c->dsum.ensure(K));
     std::vector<double> res(K);
     for (int g0 = 0; g0 < K; g0 += MKDE_LM_K) {
         const int kg = std::min(MKDE_LM_K, K - g0);
-        std::vector<double> a(kg), t(kg);
+        // the pass's kg candidates (ascending) in slots MKDE_LM_K - kg ..
+        // MKDE_LM_K - 1; the slots below hold a threshold no pair passes
+        std::vector<double> a(MKDE_LM_K, 0.0), t(MKDE_LM_K, -1.0);
         for (int j = 0; j < kg; ++j) {
             const float hk = h_cand[ord[g0 + j]];
             const double h2 = (double)(float)(hk * hk);
-            a[j] = -1.4426950408889634073599 / (2.0 * h2);
-            t[j] = 9.0 * h2;
+            a[MKDE_LM_K - kg + j] = -1.4426950408889634073599 / (2.0 * h2);
+            t[MKDE_LM_K - kg + j] = 9.0 * h2;
             if (ec_on) CHK(run_ec(c, nullptr, hk, c->ewk.p, nullptr));
             else HIPCHK(mkde_dev::fill_f32(c->stream, n, 1.0f, c->ewk.p));
-            HIPCHK(mkde_dev::coef_k(c->stream, n, c->perm.p, c->w.p, c->ewk.p, h2, c->coefk.p + j, MKDE_LM_K));
+            // candidate j of this pass in coefficient slot MKDE_LM_K - kg + j
+            HIPCHK(mkde_dev::coef_k(c->stream, n, c->perm.p, c->w.p, c->ewk.p, h2, c->coefk.p + (MKDE_LM_K - kg) + j,
+                                    MKDE_LM_K));
         }
-        CHK(copy_h2d(c, c->ak.p, a.data(), kg * sizeof(double)));
-        CHK(copy_h2d(c, c->tk.p, t.data(), kg * sizeof(double)));
+        CHK(copy_h2d(c, c->ak.p, a.data(), MKDE_LM_K * sizeof(double)));
+        CHK(copy_h2d(c, c->tk.p, t.data(), MKDE_LM_K * sizeof(double)));
         HIPCHK(hipEventRecord(c->ev0, c->stream));
         PointWork P;
-        CHK(plan_points(c, I, 1, 0.0f, c->tk.p, kg, MKDE_LM_K, P));
+        // the work estimate culls with the largest threshold (the top slot)
+        CHK(plan_points(c, I, 1, 0.0f, c->tk.p, MKDE_LM_K, MKDE_LM_K, P));
         HIPCHK(mkde_dev::loo_multi(c->stream, I, c->coefk.p, c->ak.p, c->tk.p, kg, c->den1k.p, P.order, P.items,
                                    c->pw_cost.p, c->pw_partials.p, P.n_items));
         HIPCHK(mkde_dev::point_merge(c->stream, n, MKDE_LM_K, kg, P.n_split, c->pw_splits.p, c->pw_partials.p,
@@ -1014,12 +1084,6 @@ extern "C" int mkde_loo_loglik(mkde_ctx *c, int K, const float *h_cand, int ec_o
         for (int j = 0; j < kg; ++j) res[ord[g0 + j]] = part[j];
     }
     for (int k = 0; k < K; ++k) logL[k] = res[k];
-    // restore bandwidths
-    HIPCHK(hipMemcpyAsync(c->h.p, hsave.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    c->h_fixed = saved_hf;
-    c->h_set = saved_hset;
-    c->pts_dirty = true;
     return MKDE_OK;
 }
 
@@ -1071,14 +1135,14 @@ extern "C" int mkde_loo_loglik_adaptive(mkde_ctx *c, int K, const float *h, cons
     if (K <= 0 || !h || !alpha || !logL) return fail(MKDE_E_ARG, "mkde_loo_loglik_adaptive: bad arguments");
     if (ec_on) CHK(need_mask(c));
     const int64_t n = c->n;
-    DevBuf<float> &hsave = c->hsave, &ewsave = c->ewsave;
-    CHK(hsave.ensure(n));
-    CHK(ewsave.ensure(n));
-    HIPCHK(hipMemcpyAsync(hsave.p, c->h.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(ewsave.p, c->ew.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-    const float saved_hf = c->h_fixed;
-    const bool saved_hset = c->h_set;
+    CHK(c->hsave.ensure(n));
+    CHK(c->ewsave.ensure(n));
     CHK(c->dsum.ensure(1));
+    HIPCHK(hipMemcpyAsync(c->hsave.p, c->h.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->ewsave.p, c->ew.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    // each candidate overwrites h and ew; the guard copies them back on return
+    BandwidthGuard guard(c);
+    guard.arrays = true;
     for (int k = 0; k < K; ++k) {
         if (!(h[k] > 0.0f) || !std::isfinite(h[k]) || !std::isfinite(alpha[k])) {
             logL[k] = -INFINITY;
@@ -1091,12 +1155,6 @@ extern "C" int mkde_loo_loglik_adaptive(mkde_ctx *c, int K, const float *h, cons
         CHK(sum_logs(c, c->den.p, 1, 1, c->dsum.p));
         CHK(copy_d2h(c, &logL[k], c->dsum.p, sizeof(double)));
     }
-    HIPCHK(hipMemcpyAsync(c->h.p, hsave.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->ew.p, ewsave.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    c->h_fixed = saved_hf;
-    c->h_set = saved_hset;
-    c->pts_dirty = true;
     return MKDE_OK;
 }
 
@@ -1194,15 +1252,18 @@ extern "C" int mkde_tile_upload(mkde_ctx *c, const mkde_grid *tile, const float 
     CHK(check_grid(tile));
     CHK(set_device(c));
     const int64_t nc = tile->ncols * tile->nrows;
-    c->tg = to_desc(tile);
-    c->tl = Tiling{};
+    // a failure below leaves the context without a tile
+    c->tile_set = false;
     c->tile_is_ec_mask = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
     CHK(c->tout.ensure(nc));
-    c->tile_has_mask = mask != nullptr;
     if (mask) {
         CHK(c->tmask.ensure(nc));
         CHK(copy_h2d(c, c->tmask.p, mask, nc * sizeof(float)));
     }
+    c->tg = to_desc(tile);
+    c->tl = Tiling{};
+    c->tile_has_mask = mask != nullptr;
     c->tile_set = true;
     return MKDE_OK;
 }
@@ -1212,9 +1273,12 @@ extern "C" int mkde_tile_from_mask(mkde_ctx *c)
     if (!c) return fail(MKDE_E_ARG, "null context");
     CHK(need_mask(c));
     CHK(set_device(c));
+    c->tile_set = false;
+    c->tile_is_ec_mask = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    CHK(c->tout.ensure(c->mg.ncols * c->mg.nrows));
     c->tg = c->mg;
     c->tl = Tiling{};
-    CHK(c->tout.ensure(c->mg.ncols * c->mg.nrows));
     c->tile_has_mask = true;
     c->tile_is_ec_mask = true;
     c->tile_set = true;
@@ -1401,25 +1465,32 @@ extern "C" int mkde_last_kernel_ms(mkde_ctx *c, const char *which, double *ms)
 
 namespace {
 // Phase times of a job: events are recorded around each phase and read once
-// at the end of the job, so timing adds no host synchronisation.
+// at the end of the job, so timing adds no host synchronisation.  The events
+// come from the context's pool (created on first use, kept until the context
+// is destroyed), so a job creates none and an early error return leaks none.
 struct PhaseTimes {
     struct Ev {
         hipEvent_t a, b;
         double *acc;
     };
+    mkde_ctx *c;
     std::vector<Ev> ev;
+    explicit PhaseTimes(mkde_ctx *ctx) : c(ctx) { c->ev_next = 0; }
+    ~PhaseTimes() { c->ev_next = 0; }
+    hipEvent_t get()
+    {
+        if (c->ev_next < c->ev_pool.size()) return c->ev_pool[c->ev_next++];
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+        c->ev_next = c->ev_pool.size();
+        return e;
+    }
     void finish()
     {
         for (const Ev &e : ev) {
             float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess && e.acc) *e.acc += ms;
-        }
-    }
-    ~PhaseTimes()
-    {
-        for (const Ev &e : ev) {
-            (void)hipEventDestroy(e.a);
-            (void)hipEventDestroy(e.b);
+            if (e.a && e.b && hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess && e.acc) *e.acc += ms;
         }
     }
 };
@@ -1430,13 +1501,13 @@ struct PhaseTimer {
     PhaseTimer(PhaseTimes &times, hipStream_t st, double *accum) : T(times), s(st)
     {
         e.acc = accum;
-        (void)hipEventCreate(&e.a);
-        (void)hipEventCreate(&e.b);
-        (void)hipEventRecord(e.a, s);
+        e.a = T.get();
+        e.b = T.get();
+        if (e.a) (void)hipEventRecord(e.a, s);
     }
     ~PhaseTimer()
     {
-        (void)hipEventRecord(e.b, s);
+        if (e.b) (void)hipEventRecord(e.b, s);
         T.ev.push_back(e);
     }
 };
@@ -1453,10 +1524,9 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
     if (o->hoption == -1) CHK(need_h(c));
     mkde_job_stats S;
     memset(&S, 0, sizeof S);
-    PhaseTimes times;
-    hipEvent_t t0, t1;
-    HIPCHK(hipEventCreate(&t0));
-    HIPCHK(hipEventCreate(&t1));
+    PhaseTimes times(c);
+    hipEvent_t t0 = times.get(), t1 = times.get();
+    if (!t0 || !t1) return fail(MKDE_E_HIP, "hipEventCreate failed");
     HIPCHK(hipEventRecord(t0, c->stream));
     if (ec_on) {
         PhaseTimer pt(times, c->stream, &S.ms_boundary);
@@ -1536,8 +1606,6 @@ extern "C" int mkde_run_job(mkde_ctx *c, const mkde_job_opts *o, mkde_job_stats 
     float tot = 0.0f;
     HIPCHK(hipEventElapsedTime(&tot, t0, t1));
     HIPCHK(hipEventElapsedTime(&c->last_density_ms, c->ev0, c->ev1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     S.ms_total = tot;
     if (st) *st = S;
     return MKDE_OK;

@@ -1274,19 +1274,17 @@ __device__ __forceinline__ int block_exscan256(int v, int *s_w, int *total)
 // ============================================= batched LOO (fixed bandwidths) ==
 //
 // Up to LM_K candidate bandwidths h_k (ascending) in one pass over point
-// pairs.  For each pair the distance is computed once; candidates are visited
-// from the largest down and the loop stops for the whole wave as soon as no
-// lane is inside the cut-off of candidate k (the cut-off is monotone in h).
-// coef_k of a source depends on k through the candidate's edge-correction
-// weight: coefk[k*n + i] (bin order).
+// pairs.  coef_k of a source depends on k through the candidate's
+// edge-correction weight; a pass of K candidates keeps candidate j in slot
+// LM_K - K + j of each source's LM_K coefficients (bin order).
 
 #define LM_K MKDE_LM_K
 
 struct LooMultiArgs {
     PointIndex idx;          // positions + bins (thr of the largest candidate)
-    const float *coefk;      // [n][MKDE_LM_K] bin order (candidate slots of a point contiguous)
-    const double *a_k;       // [K]
-    const double *thr_k;     // [K] ascending
+    const float *coefk;      // [n][MKDE_LM_K] bin order, candidate j of K in slot LM_K - K + j
+    const double *a_k;       // [LM_K] by slot (0 for unused slots)
+    const double *thr_k;     // [LM_K] by slot, ascending (-1 for unused slots)
     int K;
     float *den1;             // [K][n] bin order
     const int *order;        // work items as for the point kernel
@@ -1296,34 +1294,74 @@ struct LooMultiArgs {
     int n_items;
 };
 
-// Per-wave batched leave-one-out densities: each wave owns 64 queries in bin
-// order and walks the candidate bins of its own bounding box; a source's
-// position and its K coefficients are wave-uniform and come in by scalar
-// loads.  Candidates are visited from the largest bandwidth down and the
-// walk stops at the first one with no lane inside its (upper) cut-off.
-// FP32 screen; a 32-source chunk with any lane inside a 2^-20 band is redone
-// with the exact FP64 test.
-// Wave minimum of a non-negative float (or +inf), as its bit pattern in an
-// SGPR: rotations within each 16-lane DPP row, then the four row minima on
-// the scalar unit (the bit patterns of non-negative floats order like ints).
-__device__ __forceinline__ int wave_min_nonneg_bits(float v)
+// One source of the batched leave-one-out pass for the wave's 64 queries:
+// candidate slot k's term for a lane is exp2(d2f a_k) coef_k if the exact
+// cut-off fl64(dx^2 + dy^2) < thr_k holds, else 0, added to acc[k].  Slots are
+// visited from the largest bandwidth down and the walk ends at the first one
+// no lane is inside (the cut-off is monotone in h; unused slots sit below the
+// smallest candidate with thr = -1).  selfm has the bit of the lane whose
+// query is this source (or none): that lane gets d2f = +inf, so its terms are
+// exp2(-inf) * coef = 0 (leave-one-out).  Each slot's compare and term are
+// inline assembly so that its lane mask is consumed right after it is made
+// (left to itself the compiler forms all 16 masks up front and runs out of
+// scalar registers).
+__device__ __forceinline__ void loo_source(const PtRec *pts, const float *coefk, int si, uint64_t selfm, float qx,
+                                           float qy, float inf, const double (&thr)[LM_K], const float (&af)[LM_K],
+                                           float (&acc)[LM_K])
 {
-    int b = __float_as_int(v);
-    b = min(b, __builtin_amdgcn_mov_dpp(b, 0x128, 0xf, 0xf, false));   // row_ror:8
-    b = min(b, __builtin_amdgcn_mov_dpp(b, 0x124, 0xf, 0xf, false));   // row_ror:4
-    b = min(b, __builtin_amdgcn_mov_dpp(b, 0x122, 0xf, 0xf, false));   // row_ror:2
-    b = min(b, __builtin_amdgcn_mov_dpp(b, 0x121, 0xf, 0xf, false));   // row_ror:1
-    const int r0 = __builtin_amdgcn_readlane(b, 0), r1 = __builtin_amdgcn_readlane(b, 16);
-    const int r2 = __builtin_amdgcn_readlane(b, 32), r3 = __builtin_amdgcn_readlane(b, 48);
-    return min(min(r0, r1), min(r2, r3));
+    const PtRec p = pts[si];
+    const float4 *cp = (const float4 *)(coefk + (int64_t)si * LM_K);
+    const float4 cf0 = cp[0], cf1 = cp[1], cf2 = cp[2], cf3 = cp[3];
+    const float cf[LM_K] = {cf0.x, cf0.y, cf0.z, cf0.w, cf1.x, cf1.y, cf1.z, cf1.w,
+                            cf2.x, cf2.y, cf2.z, cf2.w, cf3.x, cf3.y, cf3.z, cf3.w};
+    const float dx = p.x - qx, dy = p.y - qy;
+    float d2f = __builtin_fmaf(dy, dy, dx * dx);
+    const double d2 = sq2(dx, dy);
+    asm("v_cndmask_b32_e64 %0, %0, %1, %2" : "+v"(d2f) : "v"(inf), "s"(selfm));
+#pragma unroll
+    for (int k = LM_K - 1; k >= 0; --k) {
+        // the first compare also takes the source's coefficients, so their
+        // scalar loads are issued together and complete before the chain
+        uint64_t in;
+        if (k == LM_K - 1)
+            asm("v_cmp_gt_f64_e64 %0, %1, %2"
+                : "=s"(in)
+                : "s"(thr[k]), "v"(d2), "s"(cf[0]), "s"(cf[1]), "s"(cf[2]), "s"(cf[3]), "s"(cf[4]), "s"(cf[5]),
+                  "s"(cf[6]), "s"(cf[7]), "s"(cf[8]), "s"(cf[9]), "s"(cf[10]), "s"(cf[11]), "s"(cf[12]),
+                  "s"(cf[13]), "s"(cf[14]), "s"(cf[15]));
+        else
+            asm("v_cmp_gt_f64_e64 %0, %1, %2" : "=s"(in) : "s"(thr[k]), "v"(d2));
+        if (in == 0) break;
+        // acc += (in ? exp2(d2f a_k) : 0) * coef_k; one wait state after the
+        // transcendental before its result is read, and the mask is at least
+        // two instructions old when v_cndmask reads it
+        float t;
+        asm("v_mul_f32_e32 %0, %2, %3\n\t"
+            "v_exp_f32_e32 %0, %0\n\t"
+            "s_nop 0\n\t"
+            "v_cndmask_b32_e64 %0, 0, %0, %4\n\t"
+            "v_fmac_f32_e32 %1, %5, %0"
+            : "=&v"(t), "+v"(acc[k])
+            : "v"(d2f), "v"(af[k]), "s"(in), "s"(cf[k]));
+    }
 }
 
-template <bool FULLK>
+// Batched leave-one-out densities of up to 16 fixed bandwidths in one pass
+// over point pairs: each wave owns 64 queries in bin order (lane = query) and
+// walks the candidate bins of its own bounding box; a source's position and
+// coefficients are wave-uniform and come in by scalar loads.  Per (query,
+// source) pair the distance is computed twice: d2f = fl32(dx^2 + dy^2) for the
+// exponent and d2 = fl64(dx^2 + dy^2), exact, for the cut-off (FP64 compares
+// run at the FP32 rate on gfx950, so exactness costs one VALU instruction per
+// candidate and no re-check).  The K candidates (ascending) occupy slots
+// 16-K .. 15.  Thresholds live in SGPR pairs, the exponent factors in VGPRs
+// (wave-uniform values), each source's coefficients in SGPRs.  The sums fold
+// into Kahan pairs every 32 sources.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_loo_multi_w(LooMultiArgs A)
 {
     const PointIndex &I = A.idx;
     const int lane = threadIdx.x & 63;
-    const int K = A.K;
+    const int off = LM_K - A.K;
     const int64_t n = I.n;
     const int64_t g = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (g >= (A.order ? (int64_t)A.n_items : (n + 63) / 64)) return;
@@ -1343,17 +1381,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
         qx = p.x;
         qy = p.y;
     }
-    // one FP32 threshold per candidate; the 2^-20 band is applied to the
-    // source's d2f instead (d2f * (1 + 2^-20) < t: inside for sure;
-    // d2f * (1 - 2^-20) >= t: outside for sure), which keeps the candidate
-    // constants within the scalar registers
-    float tf[LM_K], af[LM_K];
+    double thr[LM_K];
+    float af[LM_K];
 #pragma unroll
     for (int k = 0; k < LM_K; ++k) {
-        tf[k] = uni(k < K ? (float)A.thr_k[k] : -1.0f);
-        af[k] = uni(k < K ? (float)A.a_k[k] : 0.0f);
+        thr[k] = A.thr_k[k];
+        // a wave-uniform value copied into a VGPR: the scalar file holds the
+        // thresholds and the sources' coefficients
+        const float a = (float)A.a_k[k];
+        asm("v_mov_b32 %0, %1" : "=v"(af[k]) : "s"(a));
     }
-    const double tmax = A.thr_k[K - 1];
+    const double tmax = A.thr_k[LM_K - 1];
     const float mnx = uni(wave_min(qv ? qx : FLT_MAX)), mxx = uni(wave_max(qv ? qx : -FLT_MAX));
     const float mny = uni(wave_min(qv ? qy : FLT_MAX)), mxy = uni(wave_max(qv ? qy : -FLT_MAX));
     int bxa, bxb, bya, byb;
@@ -1362,6 +1400,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
     bxb = __builtin_amdgcn_readfirstlane(bxb);
     bya = __builtin_amdgcn_readfirstlane(bya);
     byb = __builtin_amdgcn_readfirstlane(byb);
+    const int q0i = (int)q0;   // bin-order indexes are below 2^30
+    const float inf = INFINITY;
 
     float acc[LM_K], sum[LM_K], comp[LM_K];
 #pragma unroll
@@ -1380,72 +1420,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
             const int en = ben - (o1 > win1 ? o1 - win1 : 0);
             for (int c0 = st; c0 < en; c0 += 32) {
                 const int c1 = c0 + 32 < en ? c0 + 32 : en;
-                bool band = false;
                 for (int si = c0; si < c1; ++si) {
-                    const PtRec p = I.pts[si];
-                    const float4 *cp = (const float4 *)(A.coefk + (int64_t)si * LM_K);
-                    const float4 cf0 = cp[0], cf1 = cp[1], cf2 = cp[2], cf3 = cp[3];
-                    const float cf[LM_K] = {cf0.x, cf0.y, cf0.z, cf0.w, cf1.x, cf1.y, cf1.z, cf1.w,
-                                            cf2.x, cf2.y, cf2.z, cf2.w, cf3.x, cf3.y, cf3.z, cf3.w};
-                    const float dx = p.x - qx, dy = p.y - qy;
-                    const bool self = (int64_t)si == q;
-                    if (FULLK) {
-                        // every slot holds a candidate.  The query itself
-                        // (leave-one-out) tests outside every cut-off: its
-                        // term is 0 for every candidate.  The wave minimum of
-                        // d2_maybe, as a uniform bit pattern, ends the walk:
-                        // candidates are visited from the largest down while
-                        // some lane may be inside (the same set a per-candidate
-                        // vote gives), compared on the scalar unit as the bit
-                        // patterns of non-negative floats
-                        const float d2f = self ? INFINITY : __builtin_fmaf(dy, dy, dx * dx);
-                        const float d2_in = d2f * (float)(1.0 + PD_BAND), d2_maybe = d2f * (float)(1.0 - PD_BAND);
-                        const int lbb = wave_min_nonneg_bits(d2_maybe);
-#pragma unroll
-                        for (int k = LM_K - 1; k >= 0; --k) {
-                            if (!(lbb < __float_as_int(tf[k]))) break;
-                            const bool maybe = d2_maybe < tf[k];
-                            const bool in = d2_in < tf[k];
-                            band |= maybe != in;
-                            const float e = fast_exp2(d2f * af[k]);
-                            acc[k] = __builtin_fmaf(in ? e : 0.0f, cf[k], acc[k]);
-                        }
-                    } else {
-                        // K < 16: a wave-wide vote per candidate ends the walk
-                        const float d2f = __builtin_fmaf(dy, dy, dx * dx);
-                        const float d2_in = d2f * (float)(1.0 + PD_BAND), d2_maybe = d2f * (float)(1.0 - PD_BAND);
-#pragma unroll
-                        for (int k = LM_K - 1; k >= 0; --k) {
-                            if (k < K) {
-                                const bool maybe = d2_maybe < tf[k];
-                                if (!__any(maybe)) break;
-                                const bool in = d2_in < tf[k];
-                                band |= maybe != in;
-                                const float e = fast_exp2(d2f * af[k]);
-                                acc[k] = __builtin_fmaf((in && !self) ? e : 0.0f, cf[k], acc[k]);
-                            }
-                        }
-                    }
-                }
-                if (__any(band)) {   // rare: exact FP64 cut-off for the chunk
-#pragma unroll
-                    for (int k = 0; k < LM_K; ++k) acc[k] = 0.0f;
-                    for (int si = c0; si < c1; ++si) {
-                        const PtRec p = I.pts[si];
-                        const float *cq = A.coefk + (int64_t)si * LM_K;
-                        const float dx = p.x - qx, dy = p.y - qy;
-                        const float d2f = __builtin_fmaf(dy, dy, dx * dx);
-                        const double d2 = sq2(dx, dy);
-                        const bool self = (int64_t)si == q;
-#pragma unroll
-                        for (int k = 0; k < LM_K; ++k) {
-                            if (k < K) {
-                                const bool in = d2 < A.thr_k[k] && !self;
-                                const float e = fast_exp2(d2f * af[k]);
-                                acc[k] = __builtin_fmaf(in ? e : 0.0f, cq[k], acc[k]);
-                            }
-                        }
-                    }
+                    // the lane whose query is this source, if any (scalar)
+                    const unsigned rel = (unsigned)(si - q0i);
+                    const uint64_t selfm = rel < 64u ? (uint64_t)1 << rel : (uint64_t)0;
+                    loo_source(I.pts, A.coefk, si, selfm, qx, qy, inf, thr, af, acc);
                 }
 #pragma unroll
                 for (int k = 0; k < LM_K; ++k) {
@@ -1455,14 +1434,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
             }
         }
     }
+    // outputs by candidate index k - off (rows of den1, rows of a part's partials)
     if (item.z > 1) {
         double *pp = A.partials + (int64_t)item.w * 64 * LM_K + lane;
 #pragma unroll
-        for (int k = 0; k < LM_K; ++k) pp[k * 64] = (double)sum[k] - (double)comp[k];
+        for (int k = 0; k < LM_K; ++k)
+            if (k >= off) pp[(k - off) * 64] = (double)sum[k] - (double)comp[k];
     } else {
 #pragma unroll
         for (int k = 0; k < LM_K; ++k)
-            if (qv && k < K) A.den1[(int64_t)k * n + q] = sum[k];
+            if (qv && k >= off) A.den1[(int64_t)(k - off) * n + q] = sum[k];
     }
 }
 
@@ -2461,10 +2442,7 @@ hipError_t loo_multi(hipStream_t s, const PointIndex &idx, const float *coefk, c
     LooMultiArgs A{idx, coefk, a_k, thr_k, K, den1, order, items, cost, partials, n_items};
     if (idx.n <= 0) return hipSuccess;
     const int64_t units = order ? n_items : (idx.n + 63) / 64;
-    // full 16-candidate passes (every pass of a bandwidth search) take the
-    // loop with one uniform bound per source
-    if (K == MKDE_LM_K) hipLaunchKernelGGL(k_loo_multi_w<true>, dim3((unsigned)((units + 3) / 4)), dim3(256), 0, s, A);
-    else hipLaunchKernelGGL(k_loo_multi_w<false>, dim3((unsigned)((units + 3) / 4)), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(k_loo_multi_w, dim3((unsigned)((units + 3) / 4)), dim3(256), 0, s, A);
     return hipGetLastError();
 }
 

@@ -14,9 +14,11 @@
 //       + bound of the next job  <=  budget
 //
 // where a job's bound (mkde_job_footprint) caps what it can add to its slot's
-// buffers.  When the next job does not fit and nothing runs, idle slots drop
-// their buffers.  Jobs share no device state, so each result is bit-identical
-// to the same job run alone.
+// buffers.  A job further back that fits may start while the head does not,
+// until MKDE_SKIP_MAX jobs have passed the head; then the head waits for
+// memory to free up.  When the head does not fit and nothing runs, idle slots
+// drop their buffers.  Jobs share no device state, so each result is
+// bit-identical to the same job run alone, in any order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,6 +44,9 @@ extern "C" int mkde_internal_fail(int code, const char *msg);
 namespace {
 
 using Clock = std::chrono::steady_clock;
+
+// Jobs that may start ahead of a queue head that does not fit in the budget.
+constexpr int MKDE_SKIP_MAX = 16;
 
 int sfail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 int sfail(int code, const char *fmt, ...)
@@ -92,6 +97,7 @@ struct mkde_server {
     std::deque<int64_t> queue;
     std::unordered_map<int64_t, Job> jobs;   // element references stay valid across inserts
     int64_t next_ticket = 1;
+    int head_skips = 0;   // jobs started ahead of the current queue head
     int64_t held_total = 0;
     int64_t committed_running = 0;
     int64_t peak = 0;
@@ -220,10 +226,24 @@ static void worker(mkde_server *s, int i)
         Job *job = nullptr;
         int64_t ticket = 0;
         if (!s->queue.empty()) {
-            ticket = s->queue.front();
-            Job &j = s->jobs.at(ticket);
-            if (s->held_total + s->committed_running + j.bound <= s->budget) {
-                s->queue.pop_front();
+            const int64_t room = s->budget - s->held_total - s->committed_running;
+            // the head if it fits, else the first later job that fits while
+            // the head has been passed fewer than MKDE_SKIP_MAX times
+            size_t pick = s->queue.size();
+            if (s->jobs.at(s->queue.front()).bound <= room) {
+                pick = 0;
+            } else if (s->head_skips < MKDE_SKIP_MAX) {
+                for (size_t q = 1; q < s->queue.size(); ++q)
+                    if (s->jobs.at(s->queue[q]).bound <= room) {
+                        pick = q;
+                        break;
+                    }
+            }
+            if (pick < s->queue.size()) {
+                ticket = s->queue[pick];
+                Job &j = s->jobs.at(ticket);
+                s->queue.erase(s->queue.begin() + (std::ptrdiff_t)pick);
+                s->head_skips = pick == 0 ? 0 : s->head_skips + 1;
                 j.state = RUNNING;
                 j.t_start = Clock::now();
                 j.res.slot = i;
@@ -353,17 +373,24 @@ extern "C" int mkde_server_wait(mkde_server *s, int64_t ticket, double timeout_m
 {
     if (!s || !res) return sfail(MKDE_E_ARG, "null argument");
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->jobs.find(ticket);
-    if (it == s->jobs.end()) return sfail(MKDE_E_ARG, "unknown ticket %lld", (long long)ticket);
-    Job &j = it->second;
-    auto done = [&] { return j.state == DONE; };
+    if (s->jobs.find(ticket) == s->jobs.end()) return sfail(MKDE_E_ARG, "unknown ticket %lld", (long long)ticket);
+    // the job is looked up again after every wake: another thread waiting on
+    // the same ticket may have retired it meanwhile
+    auto done = [&] {
+        auto f = s->jobs.find(ticket);
+        return f == s->jobs.end() || f->second.state == DONE;
+    };
     if (timeout_ms < 0) {
         s->cv_done.wait(lk, done);
     } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), done)) {
         return MKDE_E_PENDING;
     }
-    *res = j.res;
-    s->jobs.erase(ticket);
+    auto it = s->jobs.find(ticket);
+    if (it == s->jobs.end())
+        return sfail(MKDE_E_ARG, "ticket %lld was retired by another waiter", (long long)ticket);
+    *res = it->second.res;
+    s->jobs.erase(it);
+    s->cv_done.notify_all();   // a second waiter on this ticket wakes and reports it retired
     return MKDE_OK;
 }
 
@@ -376,7 +403,9 @@ extern "C" int mkde_server_cancel(mkde_server *s, int64_t ticket)
     Job &j = it->second;
     if (j.state == RUNNING) return sfail(MKDE_E_STATE, "job %lld is running", (long long)ticket);
     if (j.state == QUEUED) {
-        s->queue.erase(std::find(s->queue.begin(), s->queue.end(), ticket));
+        auto qit = std::find(s->queue.begin(), s->queue.end(), ticket);
+        if (qit == s->queue.begin()) s->head_skips = 0;   // a new head
+        s->queue.erase(qit);
         j.state = DONE;
         j.t_end = Clock::now();
         j.res.status = MKDE_E_CANCELLED;

@@ -28,6 +28,9 @@
  *                                into oracle/_ref) on random and adversarial inputs
  *   orc_loo_density              den0 - den1 = w*ew/(2*pi*h2f); symmetry; numpy brute force
  *   orc_pilot_density            numpy brute force; den0 >= self term
+ *   orc_pilot_density_queries    orc_pilot_density on the same queries (bit-identical)
+ *   orc_point_density_binned     orc_loo_density / orc_pilot_density (same pair counts,
+ *                                FP64 sums) on random and integer-lattice inputs
  *   orc_compgml / orc_abramson   closed forms (constant density -> h_i = h0)
  *   orc_golden_section / orc_hooke_jeeves   synthetic concave likelihoods with known maximiser
  *   orc_msvc_rand_points         the published MSVC CRT rand() sequence (41, 18467, 6334, ...)
@@ -399,6 +402,27 @@ void orc_pilot_density(int64_t n, const float *x, const float *y, const float *w
     }
 }
 
+/* The float loop of ComputeFixedDensityAtPoints (tiled:3096-3134) for a subset
+ * of query points: den0[q] for query i = q_idx[q], brute force over all n
+ * sources in ascending order, as orc_pilot_density computes it for every
+ * point (the CPU baseline times it on a sample of queries). */
+void orc_pilot_density_queries(int64_t n, const float *x, const float *y, const float *w, const float *ew, float h,
+                               int64_t nq, const int64_t *q_idx, float *den0)
+{
+    double hh2 = h2f(h);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t q = 0; q < nq; q++) {
+        const int64_t i = q_idx[q];
+        float den = 0.0f, self = 0.0f;
+        for (int64_t j = 0; j < n; j++) {
+            double d2 = orc_distance2(x[i], y[i], x[j], y[j]);
+            float k = orc_kernel(hh2, d2) * w[j] * ew[j];
+            if (j == i) self += k; else den += k;
+        }
+        den0[q] = den + self;
+    }
+}
+
 /* compGML (tiled:3807-3814): float sum of log(den0_i) (the float overload),
  * then expf(sum / n). */
 float orc_compgml(int64_t n, const float *den0)
@@ -502,6 +526,100 @@ int orc_hooke_jeeves(float h0, float alpha0, float stepH, float stepA, orc_logli
     out[1] = alpha0;
     out[2] = Lmax;
     return iteration;
+}
+
+/* ----------------------------------------------------- N7/N8, binned --- */
+/* The FP64 sums of orc_loo_density / orc_pilot_density for a subset of query
+ * points, with the sources found through a uniform grid of bins instead of a
+ * scan over all of them (the reference finds them with its k-d tree,
+ * KDtree::SearchRange KDtree.cu:471-537, as called from LogLikelihood
+ * tiled:3309-3334):
+ *   den64[q] = sum over sources i with orc_distance2(x_j, y_j, x_i, y_i) <
+ *              9*h2f(h_i) of K64(h2f(h_i), d2) * w_i * ew_i,   j = q_idx[q],
+ * excluding i == j when loo != 0 (LOO den1, tiled:3312-3347) and including it
+ * otherwise (den0 of the pilot, tiled:3096-3134).  The bins only skip sources
+ * whose bin lies farther from the query than 3*max(h)*(1 + 2^-16) + 2^-16
+ * (fl32 rounding of a difference moves it by at most one part in 2^24), so the
+ * contributing set is exactly the brute-force one; only the order of the FP64
+ * additions differs.  *n_pairs (optional) gets the number of contributing
+ * (query, source) pairs.  Pinned against orc_loo_density / orc_pilot_density
+ * (same pair counts, FP64 sums equal to 1e-12). */
+void orc_point_density_binned(int64_t n, const float *x, const float *y, const float *w, const float *ew,
+                              const float *h, int loo, int64_t nq, const int64_t *q_idx, double *den64,
+                              int64_t *n_pairs)
+{
+    if (n <= 0 || nq <= 0) {
+        if (n_pairs) *n_pairs = 0;
+        return;
+    }
+    double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX, hmax = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        if (x[i] < x0) x0 = x[i];
+        if (x[i] > x1) x1 = x[i];
+        if (y[i] < y0) y0 = y[i];
+        if (y[i] > y1) y1 = y[i];
+        if (h[i] > hmax) hmax = h[i];
+    }
+    const double R = ORC_SQRT_CUT * hmax * (1.0 + 1.0 / 65536.0) + 1.0 / 65536.0;
+    double size = R / 3.0;
+    const double ext = (x1 - x0 > y1 - y0 ? x1 - x0 : y1 - y0);
+    if (size < ext / 4096.0) size = ext / 4096.0;
+    if (size <= 0.0) size = 1.0;
+    const int nbx = (int)floor((x1 - x0) / size) + 1, nby = (int)floor((y1 - y0) / size) + 1;
+    const int64_t nb = (int64_t)nbx * nby;
+    int64_t *start = (int64_t *)calloc((size_t)nb + 1, sizeof(int64_t));
+    int64_t *key = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+    int64_t *order = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+    double *hh2 = (double *)malloc((size_t)n * sizeof(double));
+    for (int64_t i = 0; i < n; i++) {
+        int bx = (int)floor((x[i] - x0) / size), by = (int)floor((y[i] - y0) / size);
+        bx = bx < 0 ? 0 : (bx >= nbx ? nbx - 1 : bx);
+        by = by < 0 ? 0 : (by >= nby ? nby - 1 : by);
+        key[i] = (int64_t)by * nbx + bx;
+        start[key[i] + 1]++;
+        hh2[i] = h2f(h[i]);
+    }
+    for (int64_t b = 0; b < nb; b++) start[b + 1] += start[b];
+    {
+        int64_t *pos = (int64_t *)malloc((size_t)nb * sizeof(int64_t));
+        memcpy(pos, start, (size_t)nb * sizeof(int64_t));
+        for (int64_t i = 0; i < n; i++) order[pos[key[i]]++] = i;
+        free(pos);
+    }
+    const int k = (int)floor(R / size) + 1;
+    int64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total)
+    for (int64_t q = 0; q < nq; q++) {
+        const int64_t j = q_idx[q];
+        int bx = (int)floor((x[j] - x0) / size), by = (int)floor((y[j] - y0) / size);
+        bx = bx < 0 ? 0 : (bx >= nbx ? nbx - 1 : bx);
+        by = by < 0 ? 0 : (by >= nby ? nby - 1 : by);
+        double d64 = 0.0;
+        int64_t cnt = 0;
+        for (int cy = by - k; cy <= by + k; cy++) {
+            if (cy < 0 || cy >= nby) continue;
+            for (int cx = bx - k; cx <= bx + k; cx++) {
+                if (cx < 0 || cx >= nbx) continue;
+                const int64_t b = (int64_t)cy * nbx + cx;
+                for (int64_t t = start[b]; t < start[b + 1]; t++) {
+                    const int64_t i = order[t];
+                    if (loo && i == j) continue;
+                    const double d2 = orc_distance2(x[j], y[j], x[i], y[i]);
+                    if (d2 < ORC_CUT * hh2[i]) {
+                        d64 += orc_kernel64(hh2[i], d2) * (double)w[i] * (double)ew[i];
+                        cnt++;
+                    }
+                }
+            }
+        }
+        den64[q] = d64;
+        total += cnt;
+    }
+    free(start);
+    free(key);
+    free(order);
+    free(hh2);
+    if (n_pairs) *n_pairs = total;
 }
 
 /* --------------------------------------------------------------- N11 --- */

@@ -54,46 +54,72 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def build_native(out_dir):
+    """Compile a timing build of the same source with -O3 -march=native for
+    the host it runs on (the CPU baseline; the parity build above stays -O2,
+    portable).  Returns the library path."""
+    os.makedirs(out_dir, exist_ok=True)
+    path = os.path.join(out_dir, "libmkde_oracle_native.so")
+    subprocess.run(["gcc", "-O3", "-march=native", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-std=c11",
+                    "-shared", "-fPIC", "-o", path, os.path.join(HERE, "mkde_oracle.c"), "-lm"], check=True)
+    return path
+
+
+def use_library(path):
+    """Load the oracle from another build of mkde_oracle.c (e.g. build_native)."""
+    global _lib
+    _lib = _load(path)
+    return _lib
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        sig = {
-            "orc_version": (ctypes.c_int, []),
-            "orc_col_x": (ctypes.c_float, [_GP, ctypes.c_int64]),
-            "orc_row_y": (ctypes.c_float, [_GP, ctypes.c_int64]),
-            "orc_distance2": (ctypes.c_double, [ctypes.c_float] * 4),
-            "orc_kernel": (ctypes.c_float, [ctypes.c_double, ctypes.c_double]),
-            "orc_kernel64": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
-            "orc_rule_of_thumb": (ctypes.c_float, [ctypes.c_int64, _FP, _FP, _FP]),
-            "orc_mark_boundary": (ctypes.c_int64, [_GP, _FP, _U8P]),
-            "orc_dist_to_boundary": (None, [ctypes.c_int64, _FP, _FP, _GP, _U8P, _DP]),
-            "orc_narrow_box": (None, [_GP, ctypes.c_float, ctypes.c_float, ctypes.c_float] + [ctypes.POINTER(ctypes.c_int)] * 4),
-            "orc_ec_weights": (None, [ctypes.c_int64, _FP, _FP, _DP, _FP, _GP, _FP, _FP, _DP, _I64P]),
-            "orc_density_tile": (None, [ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _GP, _FP, _FP, _DP, _I64P]),
-            "orc_range_candidates": (ctypes.c_int64, [ctypes.c_int64, _FP, _FP, _FP, _I64P, _I32P]),
-            "orc_loo_density": (None, [ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _DP, _FP, _DP]),
-            "orc_loglik_ref32": (ctypes.c_float, [ctypes.c_int64, _FP]),
-            "orc_loglik_fp64": (ctypes.c_double, [ctypes.c_int64, _DP]),
-            "orc_pilot_density": (None, [ctypes.c_int64, _FP, _FP, _FP, _FP, ctypes.c_float, _FP, _DP]),
-            "orc_compgml": (ctypes.c_float, [ctypes.c_int64, _FP]),
-            "orc_compgml64": (ctypes.c_double, [ctypes.c_int64, _DP]),
-            "orc_abramson": (None, [ctypes.c_int64, ctypes.c_float, ctypes.c_float, ctypes.c_float, _FP, _FP]),
-            "orc_abramson64": (None, [ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _DP, _DP]),
-            "orc_golden_section": (ctypes.c_float, [ctypes.c_float, FIXED_FN, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _DP]),
-            "orc_hooke_jeeves": (ctypes.c_int, [ctypes.c_float] * 4 + [ADAPT_FN, ctypes.c_void_p, _FP]),
-            "orc_tile_point_filter": (ctypes.c_int64, [ctypes.c_int64, _FP, _FP, _FP, _GP, _I32P]),
-            "orc_msvc_rand_points": (None, [ctypes.c_int64, ctypes.c_uint32, _FP, _FP, _I32P]),
-            "orc_num_threads": (ctypes.c_int, []),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+def _load(path):
+    L = ctypes.CDLL(path)
+    sig = {
+        "orc_version": (ctypes.c_int, []),
+        "orc_col_x": (ctypes.c_float, [_GP, ctypes.c_int64]),
+        "orc_row_y": (ctypes.c_float, [_GP, ctypes.c_int64]),
+        "orc_distance2": (ctypes.c_double, [ctypes.c_float] * 4),
+        "orc_kernel": (ctypes.c_float, [ctypes.c_double, ctypes.c_double]),
+        "orc_kernel64": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+        "orc_rule_of_thumb": (ctypes.c_float, [ctypes.c_int64, _FP, _FP, _FP]),
+        "orc_mark_boundary": (ctypes.c_int64, [_GP, _FP, _U8P]),
+        "orc_dist_to_boundary": (None, [ctypes.c_int64, _FP, _FP, _GP, _U8P, _DP]),
+        "orc_narrow_box": (None, [_GP, ctypes.c_float, ctypes.c_float, ctypes.c_float] + [ctypes.POINTER(ctypes.c_int)] * 4),
+        "orc_ec_weights": (None, [ctypes.c_int64, _FP, _FP, _DP, _FP, _GP, _FP, _FP, _DP, _I64P]),
+        "orc_density_tile": (None, [ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _GP, _FP, _FP, _DP, _I64P]),
+        "orc_range_candidates": (ctypes.c_int64, [ctypes.c_int64, _FP, _FP, _FP, _I64P, _I32P]),
+        "orc_loo_density": (None, [ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _DP, _FP, _DP]),
+        "orc_loglik_ref32": (ctypes.c_float, [ctypes.c_int64, _FP]),
+        "orc_loglik_fp64": (ctypes.c_double, [ctypes.c_int64, _DP]),
+        "orc_pilot_density": (None, [ctypes.c_int64, _FP, _FP, _FP, _FP, ctypes.c_float, _FP, _DP]),
+        "orc_pilot_density_queries": (None, [ctypes.c_int64, _FP, _FP, _FP, _FP, ctypes.c_float, ctypes.c_int64,
+                                         _I64P, _FP]),
+    "orc_point_density_binned": (None, [ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, ctypes.c_int,
+                                            ctypes.c_int64, _I64P, _DP, _I64P]),
+        "orc_compgml": (ctypes.c_float, [ctypes.c_int64, _FP]),
+        "orc_compgml64": (ctypes.c_double, [ctypes.c_int64, _DP]),
+        "orc_abramson": (None, [ctypes.c_int64, ctypes.c_float, ctypes.c_float, ctypes.c_float, _FP, _FP]),
+        "orc_abramson64": (None, [ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _DP, _DP]),
+        "orc_golden_section": (ctypes.c_float, [ctypes.c_float, FIXED_FN, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _DP]),
+        "orc_hooke_jeeves": (ctypes.c_int, [ctypes.c_float] * 4 + [ADAPT_FN, ctypes.c_void_p, _FP]),
+        "orc_tile_point_filter": (ctypes.c_int64, [ctypes.c_int64, _FP, _FP, _FP, _GP, _I32P]),
+        "orc_msvc_rand_points": (None, [ctypes.c_int64, ctypes.c_uint32, _FP, _FP, _I32P]),
+        "orc_num_threads": (ctypes.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
 
 
 def _f32(a):
@@ -219,6 +245,30 @@ def pilot_density(x, y, w, ew, h):
     d0_64 = np.empty(n, np.float64)
     lib().orc_pilot_density(n, _p(x, _FP), _p(y, _FP), _p(w, _FP), _p(ew, _FP), h, _p(d0, _FP), _p(d0_64, _DP))
     return d0, d0_64
+
+
+def point_density_binned(x, y, w, ew, h, loo, queries=None):
+    """FP64 density at the query points (all points when queries is None) with
+    each source's own bandwidth, sources found through a grid of bins; loo
+    excludes the query itself.  Returns (den64[len(queries)], contributing pairs)."""
+    x, y, w, ew, h = _f32(x), _f32(y), _f32(w), _f32(ew), _f32(h)
+    n = len(x)
+    q = np.arange(n, dtype=np.int64) if queries is None else np.ascontiguousarray(queries, dtype=np.int64)
+    out = np.empty(len(q), np.float64)
+    cnt = ctypes.c_int64()
+    lib().orc_point_density_binned(n, _p(x, _FP), _p(y, _FP), _p(w, _FP), _p(ew, _FP), _p(h, _FP), int(bool(loo)),
+                                   len(q), _p(q, _I64P), _p(out, _DP), ctypes.byref(cnt))
+    return out, cnt.value
+
+
+def pilot_density_queries(x, y, w, ew, h, queries):
+    """The reference's float pilot loop (all sources) for the given queries."""
+    x, y, w, ew = _f32(x), _f32(y), _f32(w), _f32(ew)
+    q = np.ascontiguousarray(queries, dtype=np.int64)
+    out = np.empty(len(q), np.float32)
+    lib().orc_pilot_density_queries(len(x), _p(x, _FP), _p(y, _FP), _p(w, _FP), _p(ew, _FP), h, len(q), _p(q, _I64P),
+                                    _p(out, _FP))
+    return out
 
 
 def compgml(den0):

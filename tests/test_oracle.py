@@ -364,6 +364,43 @@ def test_loo_identities_and_brute_force(orc):
             orc.loglik_fp64(o["den1_64"]))
 
 
+@pytest.mark.parametrize("kind", ["random", "lattice"])
+def test_point_density_binned_matches_brute_force(orc, kind):
+    """The binned FP64 point density (used at full workload sizes) finds
+    exactly the brute-force source sets: the same pair counts as a numpy
+    brute force of the cut-off predicate, and FP64 sums equal to the
+    all-pairs oracle routines (LOO den1 and the pilot's den0) to 1e-12."""
+    r = np.random.default_rng(61)
+    n = 700
+    if kind == "random":
+        x = r.uniform(0, 60, n).astype(np.float32)
+        y = r.uniform(0, 60, n).astype(np.float32)
+        hs = [np.full(n, 2.5, np.float32), r.uniform(0.5, 6.0, n).astype(np.float32)]
+    else:
+        # integer lattice with h = 1 and 2: many pairs sit exactly on the
+        # cut-off d^2 = 9 h^2 (excluded), plus duplicates (included)
+        x = r.integers(0, 25, n).astype(np.float32)
+        y = r.integers(0, 25, n).astype(np.float32)
+        hs = [np.full(n, 1.0, np.float32), r.choice([1.0, 2.0], n).astype(np.float32)]
+    w = r.uniform(0.5, 2, n).astype(np.float32)
+    ew = r.uniform(1, 1.5, n).astype(np.float32)
+    d2 = np_d2(x[None, :], y[None, :], x[:, None], y[:, None])  # [query j, source i]
+    for h in hs:
+        h2 = (h * h).astype(np.float64)
+        inside = d2 < 9 * h2[None, :]
+        o = orc.loo_density(x, y, w, ew, h)
+        b1, c1 = orc.point_density_binned(x, y, w, ew, h, loo=True)
+        assert c1 == int(inside.sum()) - n
+        assert np.allclose(b1, o["den1_64"], rtol=1e-12, atol=1e-300)
+        q = r.choice(n, 50, replace=False)
+        b0, c0 = orc.point_density_binned(x, y, w, ew, h, loo=False, queries=q)
+        assert c0 == int(inside[q].sum())
+        assert np.allclose(b0, o["den0_64"][q], rtol=1e-12, atol=1e-300)
+    _, p64 = orc.pilot_density(x, y, w, ew, 2.0)
+    b, _ = orc.point_density_binned(x, y, w, ew, np.full(n, 2.0, np.float32), loo=False)
+    assert np.allclose(b, p64, rtol=1e-12, atol=1e-300)
+
+
 def test_loglik_clamps_at_epsilon(orc):
     eps = np.float32(1e-14)
     assert orc.loglik_fp64(np.array([0.0, 1.0])) == math.log(float(eps))
@@ -382,6 +419,8 @@ def test_pilot_matches_loo_den0_fixed_h(orc):
     o = orc.loo_density(x, y, w, ew, np.full(n, 3.3, np.float32))
     assert np.allclose(d0_64, o["den0_64"], rtol=1e-12, atol=0)
     assert np.max(np.abs(d0 - d0_64) / d0_64) <= 1e-5
+    q = np.array([0, 17, 399, 17, 250])
+    assert np.array_equal(orc.pilot_density_queries(x, y, w, ew, 3.3, q), d0[q])
 
 
 def test_gml_and_abramson_closed_forms(orc):
