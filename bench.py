@@ -344,13 +344,29 @@ def leg_search(args, with_oracle=True, O_cpu=None):
         ctx.mask_upload(M.grid(side, side, 0.0, 0.0, 1.0), mask)
         h0 = ctx.BandWidth_rule_of_thumb()
         hc = search_candidates(h0)
-        for _ in range(max(1, args.warmup // 2)):
+        def timed(reps=3):
             ctx.LogLikelihood(hc, edge_correction=True)
-        t = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            L = ctx.LogLikelihood(hc, edge_correction=True)
-            t.append(time.perf_counter() - t0)
+            tt = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                LL = ctx.LogLikelihood(hc, edge_correction=True)
+                tt.append(time.perf_counter() - t0)
+            return LL, 1e3 * float(np.median(tt))
+
+        # the host-sized launches (a host wait per candidate and per pass),
+        # then the search as one captured graph with the default and with the
+        # autotuned launch shapes
+        ctx.set_option("search_graph", 0)
+        L_host, ms_host = timed()
+        ctx.set_option("search_graph", 1)
+        L, ms_default = timed()
+        res["graph_identical_to_host_sized"] = bool(np.array_equal(L, L_host))
+        res["ms_per_search_host_sized"] = ms_host
+        res["ms_per_search_default_shapes"] = ms_default
+        if not args.no_tune:
+            res["autotune"] = ctx.autotune_search(hc, edge_correction=True, reps=3)
+        L, ms_tuned = timed()
+        t = [ms_tuned * 1e-3]
         k = int(np.argmax(L))
         e_loo = e_ec = 0
         for hk in hc:
@@ -699,6 +715,12 @@ def bench_adaptive(args):
     def step():
         return ctx.run_job(hoption=2, edge_correction=True, optimise=False, alpha=-0.5, recompute_boundary=True)
 
+    tune = None
+    if not args.no_tune:
+        # the autotuner picks the launch shapes for this job class on this GPU
+        # (untimed, before the warm-up); the timed steps run with its choice
+        tune = ctx.autotune_job(hoption=2, edge_correction=True, optimise=False, alpha=-0.5, reps=3)
+        log("autotune: %.1f -> %.1f ms, %s" % (tune["baseline_ms"], tune["final_ms"], tune["chosen"]))
     for i in range(args.warmup):
         st = step()
         log("warmup %d: %.1f ms" % (i, st["ms_total"]))
@@ -775,6 +797,7 @@ def bench_adaptive(args):
         "density_kernel": {"evals_per_s": e_den / (ker_ms * 1e-3), "cells_per_s": valid_cells / (ker_ms * 1e-3),
                            "ms": ker_ms},
         "roofline": roofline(e_den, ker_ms, rates),
+        "autotune": tune,
     }
     if dstats is not None:
         out["density_kernel"]["classification"] = dstats
@@ -812,6 +835,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the oracle checks")
     ap.add_argument("--no-legs", action="store_true", help="skip the search and tiles workloads")
+    ap.add_argument("--no-tune", action="store_true", help="run with the default launch shapes (no autotuner)")
     ap.add_argument("--stats", action="store_true", help="report density-kernel classification counters")
     ap.add_argument("--jobs", type=int, default=120, help="--workload serve: number of jobs")
     ap.add_argument("--serve-jobs", type=int, default=40,

@@ -221,6 +221,28 @@ int mkde_density_raster(mkde_ctx *ctx, int64_t ncols, int64_t nrows, const doubl
  * cell; enable == 0 reads them and switches them off. */
 int mkde_density_stats(mkde_ctx *ctx, int enable, int64_t *counters);
 
+/* Per-context options (no reference counterpart; measurement and tuning):
+ *   "search_graph"   1 (default): a leave-one-out search runs as one captured
+ *                    HIP graph, replayed while the resident inputs and K stay
+ *                    the same; 0: the host-sized launches (same results)
+ *   "density_order"  1 (default): density blocks visited heaviest first with
+ *                    heavy blocks split; 0: raster order, no split
+ *   "points_per_bin" target points per bin of the point index (default 32),
+ *                    used by the next mkde_points_upload
+ *   "ec_chunk"       512, 1024 (default) or 2048 columns per EC prefix chunk
+ *   "ec_wide_min"    boxes wider than this (default: ec_chunk) use the wide
+ *                    EC kernel
+ *   "loo_split_target", "pts_split_target", "density_split_target"
+ *                    split targets of the leave-one-out, point-density and
+ *                    density-raster work lists (defaults 32768, 16384, 4096)
+ * Split targets and chunking group the FP32 partial sums differently, so
+ * results can move by a few ulps (never the cut-off sets).  Defaults can be
+ * set from the environment (MKDE_EC_CHUNK, MKDE_EC_WIDE_MIN, MKDE_LOO_TARGET,
+ * MKDE_PTS_TARGET, MKDE_DENSITY_TARGET, MKDE_PTS_PER_BIN).  Unknown names
+ * return MKDE_E_ARG, values out of range MKDE_E_RANGE. */
+int mkde_set_option(mkde_ctx *ctx, const char *name, int64_t value);
+int mkde_get_option(mkde_ctx *ctx, const char *name, int64_t *value);
+
 /* Kernel timing of the last density launch set on the context's stream
  * (milliseconds, HIP events around the dominant kernel). */
 int mkde_last_kernel_ms(mkde_ctx *ctx, const char *which, double *ms);
@@ -265,6 +287,22 @@ int mkde_run_job(mkde_ctx *ctx, const mkde_job_opts *opts, mkde_job_stats *stats
 /* Free / total device memory of the context's GPU (the reference sizes its
  * raster tiles from cudaMemGetInfo, tiled:1152-1165). */
 int mkde_device_memory(mkde_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
+
+/* Autotuner (no reference counterpart): coordinate descent over the options
+ * above on the resident workload, timed on this device (best of reps runs per
+ * value; a value is kept when it is more than 1% faster than the current
+ * choice).  mkde_autotune_job times mkde_run_job(opts) (needs points, mask and
+ * tile); mkde_autotune_search times mkde_loo_loglik(K, h_cand, ec_on).  The
+ * context keeps the chosen options, which are also recorded per job class
+ * (log2 buckets of the point count and raster cells, bandwidth option --
+ * MKDE_H_SEARCH for searches --, edge correction, optimiser);
+ * mkde_autotune_lookup applies a recorded choice to a context (MKDE_E_STATE
+ * if the class has none).  report (may be NULL) receives a JSON text of every
+ * timing and the choices; MKDE_E_RANGE if report_cap is too small. */
+int mkde_autotune_job(mkde_ctx *ctx, const mkde_job_opts *opts, int reps, char *report, int64_t report_cap);
+int mkde_autotune_search(mkde_ctx *ctx, int K, const float *h_cand, int ec_on, int reps, char *report,
+                         int64_t report_cap);
+int mkde_autotune_lookup(mkde_ctx *ctx, int64_t n, int64_t cells, int hoption, int edge_correction, int optimise);
 
 /* ------------------------------------------------------------------- I/O --
  * GeoTIFF and CSV I/O of the command line (the reference uses GDAL through

@@ -9,6 +9,7 @@ libmkde.so on the GPU; there is no CPU fallback: if the library or a GPU is
 missing, the calls raise.
 """
 import ctypes
+import json
 import os
 
 import numpy as np
@@ -192,6 +193,11 @@ SIGNATURES = {
     "mkde_density_raster": [_CTX, ctypes.c_int64, ctypes.c_int64, _DP, ctypes.c_float, ctypes.c_int64,
                             ctypes.c_int64, _FP, _FP],
     "mkde_last_kernel_ms": [_CTX, ctypes.c_char_p, _DP],
+    "mkde_set_option": [_CTX, ctypes.c_char_p, ctypes.c_int64],
+    "mkde_autotune_job": [_CTX, ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64],
+    "mkde_autotune_search": [_CTX, ctypes.c_int, _FP, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64],
+    "mkde_autotune_lookup": [_CTX, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "mkde_get_option": [_CTX, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)],
     "mkde_run_job": [_CTX, ctypes.POINTER(JobOpts), ctypes.POINTER(JobStats)],
     "mkde_density_stats": [_CTX, ctypes.c_int, _I64P],
     "mkde_device_memory": [_CTX, _I64P, _I64P],
@@ -499,6 +505,42 @@ class Context:
         fr, to = ctypes.c_int64(), ctypes.c_int64()
         _check(_lib.mkde_device_memory(self._h, ctypes.byref(fr), ctypes.byref(to)))
         return fr.value, to.value
+
+    def set_option(self, name, value):
+        """mkde_set_option: "search_graph", "density_order", "points_per_bin"."""
+        _check(_lib.mkde_set_option(self._h, name.encode(), int(value)))
+
+    def autotune_job(self, hoption=2, edge_correction=True, optimise=False, alpha=-0.5, recompute_boundary=True,
+                     reps=3):
+        """mkde_autotune_job on the resident points / mask / tile; returns the
+        report (dict) and leaves the chosen options set."""
+        o = JobOpts(int(hoption), 1 if edge_correction else 0, 1 if optimise else 0, float(alpha),
+                    1 if recompute_boundary else 0)
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(_lib.mkde_autotune_job(self._h, ctypes.cast(ctypes.byref(o), ctypes.c_void_p), int(reps), buf,
+                                      len(buf)))
+        return json.loads(buf.value.decode())
+
+    def autotune_search(self, h_candidates, edge_correction=True, reps=3):
+        hc = _f32(np.atleast_1d(h_candidates))
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(_lib.mkde_autotune_search(self._h, len(hc), _p(hc, _FP), 1 if edge_correction else 0, int(reps), buf,
+                                         len(buf)))
+        return json.loads(buf.value.decode())
+
+    def autotune_lookup(self, n, cells, hoption, edge_correction=True, optimise=False):
+        """True when a recorded choice for the job class was applied."""
+        st = _lib.mkde_autotune_lookup(self._h, int(n), int(cells), int(hoption), 1 if edge_correction else 0,
+                                       1 if optimise else 0)
+        if st == MKDE_E_STATE:
+            return False
+        _check(st)
+        return True
+
+    def get_option(self, name):
+        v = ctypes.c_int64()
+        _check(_lib.mkde_get_option(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
 
     def last_kernel_ms(self, which="density"):
         v = ctypes.c_double()

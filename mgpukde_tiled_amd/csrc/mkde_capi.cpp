@@ -9,6 +9,10 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <functional>
+#include <map>
+#include <mutex>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
@@ -29,19 +33,20 @@ hipError_t prep_points(hipStream_t, int64_t, const int *, const float *, const f
 hipError_t bin_stats(hipStream_t, int, const int *, const PtRec *, BinInfo *, double *);
 int64_t raster_blocks(const GridDesc &, const Tiling &);
 hipError_t density_plan(hipStream_t, const GridDesc &, const float *, const PointIndex &, const Tiling &,
-                        const DensityWork &);
-hipError_t density_items(hipStream_t, int, const DensityWork &, int, int4 *, int4 *, int *);
+                        const DensityWork &, const Tuning &);
+hipError_t density_items(hipStream_t, int, const DensityWork &, int, int4 *, int4 *, int *, const unsigned *);
 hipError_t density_tile(hipStream_t, const GridDesc &, const float *, float *, const PointIndex &,
                         unsigned long long *, const int *, const int4 *, const unsigned *, double *, int,
                         const Tiling &);
 hipError_t density_merge(hipStream_t, const GridDesc &, const Tiling &, const float *, int, const int4 *,
                          const double *, float *);
-hipError_t point_plan(hipStream_t, const PointIndex &, int, float, const double *, int, const DensityWork &);
+hipError_t point_plan(hipStream_t, const PointIndex &, int, float, const double *, int, const DensityWork &,
+                      const Tuning &);
 hipError_t point_density(hipStream_t, const PointIndex &, int, float *, float, const int *, const int4 *,
                          const unsigned *, double *, int);
 hipError_t loo_multi(hipStream_t, const PointIndex &, const float *, const double *, const double *, int, float *,
-                     const int *, const int4 *, const unsigned *, double *, int);
-hipError_t point_merge(hipStream_t, int64_t, int, int, int, const int4 *, const double *, float *);
+                     const int *, const int4 *, const unsigned *, double *, int, const unsigned *);
+hipError_t point_merge(hipStream_t, int64_t, int, int, int, const int4 *, const double *, float *, const unsigned *);
 hipError_t range_count(hipStream_t, const PointIndex &, int64_t *);
 hipError_t range_fill(hipStream_t, const PointIndex &, const int64_t *, int32_t *);
 hipError_t valid_cells(hipStream_t, int64_t, const float *, float, uint8_t *);
@@ -51,13 +56,14 @@ hipError_t bt_build(hipStream_t, const GridDesc &, const uint8_t *, uint4 *, flo
 hipError_t dist_tree(hipStream_t, int64_t, const float *, const float *, const int *, const GridDesc &,
                      const BndTree &, double *);
 hipError_t ec_active(hipStream_t, int64_t, const GridDesc &, const float *, const float *, const double *,
-                     const float *, float, float *, int *, int *, int, int);
-int ec_chunk();
-int ec_wide_min();
+                     const float *, float, const float *, float *, int *, int *, int, int);
 int ec_wide_max();
+Tuning default_tuning();
 hipError_t centres(hipStream_t, const GridDesc &, float *, float *);
 hipError_t ec(hipStream_t, int64_t, int, int, int, const GridDesc &, const MaskRuns &, const float *, const float *,
-              const float *, const float *, const float *, float, const int *, float *, unsigned long long *);
+              const float *, const float *, const float *, float, const int *, float *, unsigned long long *, int);
+hipError_t ec_dev(hipStream_t, int64_t, const GridDesc &, const MaskRuns &, const float *, const float *, const float *,
+                  const float *, const float *, const int *, const int *, float *, int);
 hipError_t count_pairs(hipStream_t, const GridDesc &, const MaskRuns &, const PointIndex &, unsigned long long *);
 int64_t slot_words();
 hipError_t fold_slots(hipStream_t, unsigned long long *);
@@ -68,6 +74,7 @@ hipError_t abramson(hipStream_t, int64_t, const int *, const float *, const doub
 hipError_t unsort_f32(hipStream_t, int64_t, const int *, const float *, float *);
 hipError_t sort_f32(hipStream_t, int64_t, const int *, const float *, float *);
 hipError_t coef_k(hipStream_t, int64_t, const int *, const float *, const float *, double, float *, int);
+hipError_t coef_k_dev(hipStream_t, int64_t, const int *, const float *, const float *, const float *, float *, int);
 hipError_t fill_f32(hipStream_t, int64_t, float, float *);
 }  // namespace mkde_dev
 
@@ -171,6 +178,7 @@ struct mkde_ctx {
     DevBuf<double> bin_thr;
     DevBuf<double> reach;       // 1 value
     bool pts_dirty = true;
+    Tuning tune{};              // launch shapes (mkde_set_option, autotuner)
 
     // per-job arrays, original order
     DevBuf<float> h, ew;
@@ -216,6 +224,15 @@ struct mkde_ctx {
     DevBuf<float> coefk, ewk, den1k;
     DevBuf<double> ak, tk;
     DevBuf<float> hsave, ewsave;   // bandwidths / EC weights saved across a likelihood scan
+    // the batched bandwidth search as one captured graph (mkde_loo_loglik):
+    // candidate bandwidths, slot constants and log-likelihoods on the device,
+    // the graph and the key of everything it captured
+    bool search_graph_on = true;
+    DevBuf<float> s_h;          // K candidates, ascending
+    DevBuf<double> s_a, s_t;    // 16 slot constants per pass
+    DevBuf<double> s_logl;      // K
+    hipGraphExec_t s_exec = nullptr;
+    std::vector<uint64_t> s_key;
     // work list of the point kernels (64-query waves, heavy ones split)
     DevBuf<unsigned> pw_cost, pw_parts, pw_item0, pw_slot0, pw_sidx, pw_sidx2, pw_chunks, pw_counts, pw_hist;
     DevBuf<unsigned long long> pw_total;
@@ -266,11 +283,14 @@ extern "C" int mkde_ctx_create(int device, mkde_ctx **out)
     mkde_ctx *c = new (std::nothrow) mkde_ctx();
     if (!c) return fail(MKDE_E_NOMEM, "out of host memory");
     c->device = device;
+    c->tune = mkde_dev::default_tuning();
     // tuning knob: MKDE_DENSITY_ORDER=0 visits density blocks in raster order
     const char *ord = getenv("MKDE_DENSITY_ORDER");
     if (ord && ord[0] == '0') c->dt_order_on = false;
     const char *ppb = getenv("MKDE_PTS_PER_BIN");
     if (ppb && atoi(ppb) > 0) c->pts_per_bin = atoi(ppb);
+    const char *sg = getenv("MKDE_SEARCH_GRAPH");
+    if (sg && sg[0] == '0') c->search_graph_on = false;
     e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
@@ -290,6 +310,7 @@ extern "C" int mkde_ctx_destroy(mkde_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->s_exec) (void)hipGraphExecDestroy(c->s_exec);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -315,7 +336,11 @@ extern "C" int mkde_ctx_release(mkde_ctx *c)
     hipEvent_t e0 = c->ev0, e1 = c->ev1;
     std::vector<hipEvent_t> pool;
     pool.swap(c->ev_pool);
-    const bool order_on = c->dt_order_on;
+    const bool order_on = c->dt_order_on, graph_on = c->search_graph_on;
+    const Tuning tune = c->tune;
+    const int ppb = c->pts_per_bin;
+    if (c->s_exec) (void)hipGraphExecDestroy(c->s_exec);
+    c->s_exec = nullptr;
     c->~mkde_ctx();
     new (c) mkde_ctx();
     c->device = dev;
@@ -324,6 +349,9 @@ extern "C" int mkde_ctx_release(mkde_ctx *c)
     c->ev1 = e1;
     c->ev_pool.swap(pool);
     c->dt_order_on = order_on;
+    c->search_graph_on = graph_on;
+    c->tune = tune;
+    c->pts_per_bin = ppb;
     t_acct = &c->dev_bytes;
     return MKDE_OK;
 }
@@ -824,14 +852,15 @@ static int run_ec(mkde_ctx *c, const float *h, float hf, float *ew_out, unsigned
     CHK(c->ucount.ensure(nw));
     HIPCHK(hipMemsetAsync(c->icount.p, 0, 3 * sizeof(int), c->stream));
     HIPCHK(hipMemsetAsync(c->ucount.p, 0, nw * sizeof(unsigned long long), c->stream));
-    HIPCHK(mkde_dev::ec_active(c->stream, c->n, c->mg, c->x.p, c->y.p, c->dist.p, h, hf, ew_out, c->list.p,
-                               c->icount.p, mkde_dev::ec_wide_min(), mkde_dev::ec_wide_max()));
+    HIPCHK(mkde_dev::ec_active(c->stream, c->n, c->mg, c->x.p, c->y.p, c->dist.p, h, hf, nullptr, ew_out, c->list.p,
+                               c->icount.p, std::min(c->tune.ec_wide_min, mkde_dev::ec_wide_max()),
+                               mkde_dev::ec_wide_max()));
     int cnt3[3] = {0, 0, 0};   // narrow points, wide points, widest box
     CHK(copy_d2h(c, cnt3, c->icount.p, sizeof cnt3));
     MaskRuns R{c->run_ptr.p, c->run_se.p};
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     HIPCHK(mkde_dev::ec(c->stream, c->n, cnt3[0], cnt3[1], cnt3[2], c->mg, R, c->colx.p, c->rowy.p, c->x.p, c->y.p,
-                        h, hf, c->list.p, ew_out, c->ucount.p));
+                        h, hf, c->list.p, ew_out, c->ucount.p, c->tune.ec_chunk));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     // the pair count: folded, and read back unless the caller reads it later
     // (fold_count: from the last word of ucount, after its own wait)
@@ -943,7 +972,7 @@ static int plan_points(mkde_ctx *c, const PointIndex &I, int kind, float h_fixed
     CHK(c->pw_total.ensure(1));
     DensityWork W{c->pw_cost.p,  c->pw_parts.p,  c->pw_item0.p, c->pw_slot0.p, c->pw_sidx.p,
                   c->pw_sidx2.p, c->pw_chunks.p, c->pw_total.p, c->pw_counts.p, c->pw_hist.p};
-    HIPCHK(mkde_dev::point_plan(c->stream, I, kind, h_fixed, thr_k, K, W));
+    HIPCHK(mkde_dev::point_plan(c->stream, I, kind, h_fixed, thr_k, K, W, c->tune));
     unsigned cnt[3];
     CHK(copy_d2h(c, cnt, c->pw_counts.p, sizeof cnt));
     P.n_items = (int)cnt[0];
@@ -952,7 +981,7 @@ static int plan_points(mkde_ctx *c, const PointIndex &I, int kind, float h_fixed
     CHK(c->pw_order.ensure(P.n_items));
     CHK(c->pw_splits.ensure(P.n_split));
     CHK(c->pw_partials.ensure((size_t)cnt[1] * 64 * m));
-    HIPCHK(mkde_dev::density_items(c->stream, (int)nw, W, P.n_items, c->pw_items.p, c->pw_splits.p, c->pw_order.p));
+    HIPCHK(mkde_dev::density_items(c->stream, (int)nw, W, P.n_items, c->pw_items.p, c->pw_splits.p, c->pw_order.p, nullptr));
     P.order = c->pw_order.p;
     P.items = c->pw_items.p;
     return MKDE_OK;
@@ -967,7 +996,7 @@ static int density_points_sorted(mkde_ctx *c, int loo, float *den_sorted)
     CHK(plan_points(c, I, 0, c->h_fixed, nullptr, 0, 1, P));
     HIPCHK(mkde_dev::point_density(c->stream, I, loo, den_sorted, c->h_fixed, P.order, P.items, c->pw_cost.p,
                                    c->pw_partials.p, P.n_items));
-    HIPCHK(mkde_dev::point_merge(c->stream, c->n, 1, 1, P.n_split, c->pw_splits.p, c->pw_partials.p, den_sorted));
+    HIPCHK(mkde_dev::point_merge(c->stream, c->n, 1, 1, P.n_split, c->pw_splits.p, c->pw_partials.p, den_sorted, nullptr));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     return MKDE_OK;
 }
@@ -1020,6 +1049,177 @@ struct BandwidthGuard {
     }
 };
 
+// ----------------------------------------------- the search as a graph --
+// A bandwidth search of K candidates runs as one HIP graph: for every
+// candidate the EC pass (active list, narrow and wide kernels, all sized on
+// the device) and its coefficient slot, then per 16 candidates the work list,
+// the batched leave-one-out pass, the merge of split waves and the FP64 sums
+// of logs into s_logl -- no host wait between the first launch and the
+// read-back of the K log-likelihoods.  Buffers are sized for the largest
+// work list a pass can have (waves + split target items and split parts), so
+// nothing depends on a value only the device knows.  The candidates'
+// bandwidths and slot constants are read from device arrays written before
+// each launch, so the graph captured once serves every later search on the
+// same resident inputs with the same K (the key lists every pointer and size
+// the graph holds).  The kernels and their order are those of the host-sized
+// path, so the log-likelihoods are the same bit for bit.
+static const int64_t MKDE_SEARCH_GRAPH_MAX_PARTIALS = (int64_t)1 << 30;   // bytes of split-part sums
+
+static int search_enqueue(mkde_ctx *c, int K, int ec_on, const PointIndex &I, int64_t items_max, int64_t nw)
+{
+    const int64_t n = c->n;
+    MaskRuns R{c->run_ptr.p, c->run_se.p};
+    DensityWork W{c->pw_cost.p,  c->pw_parts.p,  c->pw_item0.p, c->pw_slot0.p, c->pw_sidx.p,
+                  c->pw_sidx2.p, c->pw_chunks.p, c->pw_total.p, c->pw_counts.p, c->pw_hist.p};
+    if (!ec_on) HIPCHK(mkde_dev::fill_f32(c->stream, n, 1.0f, c->ewk.p));
+    const int nb = (int)std::min<int64_t>(1024, (n + 255) / 256);
+    for (int g0 = 0, gi = 0; g0 < K; g0 += MKDE_LM_K, ++gi) {
+        const int kg = std::min(MKDE_LM_K, K - g0);
+        for (int j = 0; j < kg; ++j) {
+            const float *hd = c->s_h.p + g0 + j;
+            if (ec_on) {
+                HIPCHK(mkde_dev::fill_f32(c->stream, 3, 0.0f, (float *)c->icount.p));   // zero counts (int bits)
+                HIPCHK(mkde_dev::ec_active(c->stream, n, c->mg, c->x.p, c->y.p, c->dist.p, nullptr, 0.0f, hd,
+                                           c->ewk.p, c->list.p, c->icount.p,
+                                           std::min(c->tune.ec_wide_min, mkde_dev::ec_wide_max()),
+                                           mkde_dev::ec_wide_max()));
+                HIPCHK(mkde_dev::ec_dev(c->stream, n, c->mg, R, c->colx.p, c->rowy.p, c->x.p, c->y.p, hd,
+                                        c->icount.p, c->list.p, c->ewk.p, c->tune.ec_chunk));
+            }
+            HIPCHK(mkde_dev::coef_k_dev(c->stream, n, c->perm.p, c->w.p, c->ewk.p, hd,
+                                        c->coefk.p + (MKDE_LM_K - kg) + j, MKDE_LM_K));
+        }
+        const double *a = c->s_a.p + (int64_t)gi * MKDE_LM_K, *t = c->s_t.p + (int64_t)gi * MKDE_LM_K;
+        HIPCHK(mkde_dev::point_plan(c->stream, I, 1, 0.0f, t, MKDE_LM_K, W, c->tune));
+        HIPCHK(mkde_dev::density_items(c->stream, (int)nw, W, (int)items_max, c->pw_items.p, c->pw_splits.p,
+                                       c->pw_order.p, c->pw_counts.p));
+        HIPCHK(mkde_dev::loo_multi(c->stream, I, c->coefk.p, a, t, kg, c->den1k.p, c->pw_order.p, c->pw_items.p,
+                                   c->pw_cost.p, c->pw_partials.p, (int)items_max, c->pw_counts.p));
+        HIPCHK(mkde_dev::point_merge(c->stream, n, MKDE_LM_K, kg, (int)nw, c->pw_splits.p, c->pw_partials.p,
+                                     c->den1k.p, c->pw_counts.p + 2));
+        HIPCHK(mkde_dev::sumlog(c->stream, n, kg, c->den1k.p, 1, c->partial.p, nb, c->s_logl.p + g0));
+    }
+    return MKDE_OK;
+}
+
+// Runs the search as a graph when its buffers fit; *done = false leaves the
+// search to the host-sized path.
+static int loo_loglik_graph(mkde_ctx *c, int K, const std::vector<int> &ord, const float *h_cand, int ec_on,
+                            double *logL, bool *done)
+{
+    *done = false;
+    const int64_t n = c->n, nw = (n + 63) / 64;
+    int64_t items_max = 0, slots_max = 0;
+    point_work_bounds(n, c->tune.loo_target, &items_max, &slots_max);
+    if (slots_max * 64 * MKDE_LM_K * (int64_t)sizeof(double) > MKDE_SEARCH_GRAPH_MAX_PARTIALS) return MKDE_OK;
+    const int G = (K + MKDE_LM_K - 1) / MKDE_LM_K;
+    // every buffer the graph touches, at its final size, before any capture
+    if (ec_on) {
+        CHK(ensure_dist(c));
+        CHK(c->list.ensure(n));
+        CHK(c->icount.ensure(3));
+    }
+    CHK(c->ewk.ensure(n));
+    CHK(c->coefk.ensure((size_t)MKDE_LM_K * n));
+    CHK(c->den1k.ensure((size_t)MKDE_LM_K * n));
+    CHK(c->s_h.ensure(K));
+    CHK(c->s_a.ensure((size_t)G * MKDE_LM_K));
+    CHK(c->s_t.ensure((size_t)G * MKDE_LM_K));
+    CHK(c->s_logl.ensure(K));
+    CHK(c->pw_cost.ensure(nw));
+    CHK(c->pw_parts.ensure(nw));
+    CHK(c->pw_item0.ensure(nw));
+    CHK(c->pw_slot0.ensure(nw));
+    CHK(c->pw_sidx.ensure(nw));
+    CHK(c->pw_sidx2.ensure(nw));
+    CHK(c->pw_chunks.ensure((nw + 1023) / 1024));
+    CHK(c->pw_counts.ensure(3));
+    CHK(c->pw_hist.ensure(32));
+    CHK(c->pw_total.ensure(1));
+    CHK(c->pw_items.ensure(items_max));
+    CHK(c->pw_order.ensure(items_max));
+    CHK(c->pw_splits.ensure(nw));
+    CHK(c->pw_partials.ensure((size_t)slots_max * 64 * MKDE_LM_K));
+    const int nb = (int)std::min<int64_t>(1024, (n + 255) / 256);
+    CHK(c->partial.ensure((size_t)nb * MKDE_LM_K));
+    // the device inputs of this search: candidates ascending, slot constants
+    std::vector<float> hs(K);
+    std::vector<double> a((size_t)G * MKDE_LM_K, 0.0), t((size_t)G * MKDE_LM_K, -1.0);
+    for (int k = 0; k < K; ++k) hs[k] = h_cand[ord[k]];
+    for (int g0 = 0, gi = 0; g0 < K; g0 += MKDE_LM_K, ++gi) {
+        const int kg = std::min(MKDE_LM_K, K - g0);
+        for (int j = 0; j < kg; ++j) {
+            const double h2 = (double)(float)(hs[g0 + j] * hs[g0 + j]);
+            a[(size_t)gi * MKDE_LM_K + MKDE_LM_K - kg + j] = -1.4426950408889634073599 / (2.0 * h2);
+            t[(size_t)gi * MKDE_LM_K + MKDE_LM_K - kg + j] = 9.0 * h2;
+        }
+    }
+    CHK(copy_h2d(c, c->s_h.p, hs.data(), K * sizeof(float)));
+    CHK(copy_h2d(c, c->s_a.p, a.data(), a.size() * sizeof(double)));
+    CHK(copy_h2d(c, c->s_t.p, t.data(), t.size() * sizeof(double)));
+    PointIndex I = index_of(c);
+    const uint64_t key_v[] = {(uint64_t)n, (uint64_t)K, (uint64_t)ec_on, (uint64_t)c->nbins,
+                              (uint64_t)(uintptr_t)I.pts, (uint64_t)(uintptr_t)I.bin_start,
+                              (uint64_t)(uintptr_t)I.bin_box, (uint64_t)(uintptr_t)I.bin_thr,
+                              (uint64_t)(uintptr_t)I.reach_p, (uint64_t)(uintptr_t)c->x.p,
+                              (uint64_t)(uintptr_t)c->y.p, (uint64_t)(uintptr_t)c->w.p,
+                              (uint64_t)(uintptr_t)c->perm.p, (uint64_t)(uintptr_t)c->dist.p,
+                              (uint64_t)(uintptr_t)c->mask.p, (uint64_t)(uintptr_t)c->run_ptr.p,
+                              (uint64_t)(uintptr_t)c->run_se.p, (uint64_t)(uintptr_t)c->colx.p,
+                              (uint64_t)(uintptr_t)c->rowy.p, (uint64_t)c->mg.ncols, (uint64_t)c->mg.nrows,
+                              (uint64_t)__builtin_bit_cast(uint32_t, c->mg.xll),
+                              (uint64_t)__builtin_bit_cast(uint32_t, c->mg.yll),
+                              (uint64_t)__builtin_bit_cast(uint32_t, c->mg.cs), (uint64_t)c->bg.nbx,
+                              (uint64_t)c->bg.nby, __builtin_bit_cast(uint64_t, c->bg.x0),
+                              __builtin_bit_cast(uint64_t, c->bg.y0), __builtin_bit_cast(uint64_t, c->bg.size),
+                              (uint64_t)(uintptr_t)c->list.p, (uint64_t)(uintptr_t)c->icount.p,
+                              (uint64_t)(uintptr_t)c->ewk.p, (uint64_t)(uintptr_t)c->coefk.p,
+                              (uint64_t)(uintptr_t)c->den1k.p, (uint64_t)(uintptr_t)c->s_h.p,
+                              (uint64_t)(uintptr_t)c->s_a.p, (uint64_t)(uintptr_t)c->s_t.p,
+                              (uint64_t)(uintptr_t)c->s_logl.p, (uint64_t)(uintptr_t)c->pw_cost.p,
+                              (uint64_t)(uintptr_t)c->pw_parts.p, (uint64_t)(uintptr_t)c->pw_item0.p,
+                              (uint64_t)(uintptr_t)c->pw_slot0.p, (uint64_t)(uintptr_t)c->pw_sidx.p,
+                              (uint64_t)(uintptr_t)c->pw_sidx2.p, (uint64_t)(uintptr_t)c->pw_chunks.p,
+                              (uint64_t)(uintptr_t)c->pw_counts.p, (uint64_t)(uintptr_t)c->pw_hist.p,
+                              (uint64_t)(uintptr_t)c->pw_total.p, (uint64_t)(uintptr_t)c->pw_items.p,
+                              (uint64_t)(uintptr_t)c->pw_order.p, (uint64_t)(uintptr_t)c->pw_splits.p,
+                              (uint64_t)(uintptr_t)c->pw_partials.p, (uint64_t)(uintptr_t)c->partial.p,
+                              (uint64_t)c->tune.ec_chunk, (uint64_t)c->tune.ec_wide_min, c->tune.loo_target};
+    std::vector<uint64_t> key(std::begin(key_v), std::end(key_v));
+    if (!c->s_exec || c->s_key != key) {
+        if (c->s_exec) {
+            (void)hipGraphExecDestroy(c->s_exec);
+            c->s_exec = nullptr;
+        }
+        c->s_key.clear();
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int st = search_enqueue(c, K, ec_on, I, items_max, nw);
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (st != MKDE_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        if (e != hipSuccess) return fail(MKDE_E_HIP, "capturing the search graph failed: %s", hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&c->s_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            c->s_exec = nullptr;
+            return fail(MKDE_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+        }
+        c->s_key = key;
+    }
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(hipGraphLaunch(c->s_exec, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    std::vector<double> res(K);
+    CHK(copy_d2h(c, res.data(), c->s_logl.p, K * sizeof(double)));
+    HIPCHK(hipEventElapsedTime(&c->last_points_ms, c->ev0, c->ev1));
+    for (int k = 0; k < K; ++k) logL[ord[k]] = res[k];
+    *done = true;
+    return MKDE_OK;
+}
+
 extern "C" int mkde_loo_loglik(mkde_ctx *c, int K, const float *h_cand, int ec_on, double *logL)
 {
     CHK(need_points(c));
@@ -1041,6 +1241,11 @@ extern "C" int mkde_loo_loglik(mkde_ctx *c, int K, const float *h_cand, int ec_o
     c->h_fixed = hmax;
     c->pts_dirty = true;
     CHK(prep(c));
+    if (c->search_graph_on) {
+        bool done = false;
+        CHK(loo_loglik_graph(c, K, ord, h_cand, ec_on, logL, &done));
+        if (done) return MKDE_OK;
+    }
     PointIndex I = index_of(c);
     const int G = std::min(K, MKDE_LM_K);
     CHK(c->coefk.ensure((size_t)MKDE_LM_K * n));   // [n][MKDE_LM_K]: a point's candidate slots contiguous
@@ -1073,9 +1278,9 @@ extern "C" int mkde_loo_loglik(mkde_ctx *c, int K, const float *h_cand, int ec_o
         // the work estimate culls with the largest threshold (the top slot)
         CHK(plan_points(c, I, 1, 0.0f, c->tk.p, MKDE_LM_K, MKDE_LM_K, P));
         HIPCHK(mkde_dev::loo_multi(c->stream, I, c->coefk.p, c->ak.p, c->tk.p, kg, c->den1k.p, P.order, P.items,
-                                   c->pw_cost.p, c->pw_partials.p, P.n_items));
+                                   c->pw_cost.p, c->pw_partials.p, P.n_items, nullptr));
         HIPCHK(mkde_dev::point_merge(c->stream, n, MKDE_LM_K, kg, P.n_split, c->pw_splits.p, c->pw_partials.p,
-                                     c->den1k.p));
+                                     c->den1k.p, nullptr));
         HIPCHK(hipEventRecord(c->ev1, c->stream));
         CHK(sum_logs(c, c->den1k.p, kg, 1, c->dsum.p));
         std::vector<double> part(kg);
@@ -1330,7 +1535,7 @@ extern "C" int mkde_density_tile_resident(mkde_ctx *c)
         CHK(c->blk_total.ensure(1));
         DensityWork W{c->blk_cost.p, c->blk_parts.p,  c->blk_item0.p, c->blk_slot0.p, c->blk_sidx.p,
                       c->blk_sidx2.p, c->blk_chunks.p, c->blk_total.p, c->blk_counts.p, c->blk_hist.p};
-        HIPCHK(mkde_dev::density_plan(c->stream, c->tg, tm, I, c->tl, W));
+        HIPCHK(mkde_dev::density_plan(c->stream, c->tg, tm, I, c->tl, W, c->tune));
         unsigned cnt[3];
         CHK(copy_d2h(c, cnt, c->blk_counts.p, sizeof cnt));
         n_items = (int)cnt[0];
@@ -1340,7 +1545,7 @@ extern "C" int mkde_density_tile_resident(mkde_ctx *c)
         CHK(c->dt_splits.ensure(n_split));
         CHK(c->dt_partials.ensure((size_t)cnt[1] * MKDE_DT_CELLS));
         HIPCHK(mkde_dev::density_items(c->stream, (int)nblk, W, n_items, c->dt_items.p, c->dt_splits.p,
-                                       c->dt_order.p));
+                                       c->dt_order.p, nullptr));
         order = c->dt_order.p;
     }
     HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -1448,6 +1653,219 @@ extern "C" int mkde_density_stats(mkde_ctx *c, int enable, int64_t *counters)
         c->dt_stats_on = false;
     }
     if (counters) CHK(copy_d2h(c, counters, c->dt_stats.p, 8 * sizeof(int64_t)));
+    return MKDE_OK;
+}
+
+extern "C" int mkde_set_option(mkde_ctx *c, const char *name, int64_t value)
+{
+    if (!c || !name) return fail(MKDE_E_ARG, "null argument");
+    if (!strcmp(name, "search_graph")) {
+        c->search_graph_on = value != 0;
+    } else if (!strcmp(name, "density_order")) {
+        c->dt_order_on = value != 0;
+    } else if (!strcmp(name, "points_per_bin")) {
+        if (value < 1 || value > 4096) return fail(MKDE_E_RANGE, "points_per_bin %lld not in [1, 4096]", (long long)value);
+        c->pts_per_bin = (int)value;   // takes effect at the next mkde_points_upload
+    } else if (!strcmp(name, "ec_chunk")) {
+        if (value != 512 && value != 1024 && value != 2048) return fail(MKDE_E_RANGE, "ec_chunk must be 512, 1024 or 2048");
+        c->tune.ec_chunk = (int)value;
+    } else if (!strcmp(name, "ec_wide_min")) {
+        if (value < 1 || value > mkde_dev::ec_wide_max())
+            return fail(MKDE_E_RANGE, "ec_wide_min %lld not in [1, %d]", (long long)value, mkde_dev::ec_wide_max());
+        c->tune.ec_wide_min = (int)value;
+    } else if (!strcmp(name, "loo_split_target") || !strcmp(name, "pts_split_target") ||
+               !strcmp(name, "density_split_target")) {
+        if (value < 64 || value > ((int64_t)1 << 24)) return fail(MKDE_E_RANGE, "%s %lld not in [64, 2^24]", name, (long long)value);
+        (name[0] == 'l' ? c->tune.loo_target : name[0] == 'p' ? c->tune.pts_target : c->tune.dt_target) =
+            (unsigned long long)value;
+    } else {
+        return fail(MKDE_E_ARG, "unknown option '%s'", name);
+    }
+    return MKDE_OK;
+}
+
+extern "C" int mkde_get_option(mkde_ctx *c, const char *name, int64_t *value)
+{
+    if (!c || !name || !value) return fail(MKDE_E_ARG, "null argument");
+    if (!strcmp(name, "search_graph")) *value = c->search_graph_on ? 1 : 0;
+    else if (!strcmp(name, "density_order")) *value = c->dt_order_on ? 1 : 0;
+    else if (!strcmp(name, "points_per_bin")) *value = c->pts_per_bin;
+    else if (!strcmp(name, "ec_chunk")) *value = c->tune.ec_chunk;
+    else if (!strcmp(name, "ec_wide_min")) *value = c->tune.ec_wide_min;
+    else if (!strcmp(name, "loo_split_target")) *value = (int64_t)c->tune.loo_target;
+    else if (!strcmp(name, "pts_split_target")) *value = (int64_t)c->tune.pts_target;
+    else if (!strcmp(name, "density_split_target")) *value = (int64_t)c->tune.dt_target;
+    else return fail(MKDE_E_ARG, "unknown option '%s'", name);
+    return MKDE_OK;
+}
+
+// ------------------------------------------------------------ autotuner --
+// Coordinate descent over the launch-shape options of a context, timing a
+// resident workload on the device it runs on: each option in turn takes each
+// of its candidate values with the others at their current choice; a value
+// replaces the current one only when its best-of-reps time is more than 1%
+// faster.  The context keeps the chosen values; the choices are recorded per
+// job class (log2 buckets of the point count and raster cells, bandwidth
+// option, edge correction, optimiser) for mkde_autotune_lookup, and a JSON
+// report lists every timing.  Block width is not tuned (a build-time
+// constant, MKDE_DT_BW).
+namespace {
+struct TuneChoice {
+    Tuning t;
+    int ppb;
+};
+std::mutex g_tune_mu;
+std::map<std::vector<int64_t>, TuneChoice> g_tune_table;
+
+std::vector<int64_t> tune_class(int64_t n, int64_t cells, int hoption, int ec, int optimise)
+{
+    auto lg = [](int64_t v) {
+        int b = 0;
+        while (v > 1) {
+            v >>= 1;
+            ++b;
+        }
+        return (int64_t)b;
+    };
+    return {lg(n), lg(cells), hoption, ec ? 1 : 0, optimise ? 1 : 0};
+}
+
+struct TuneOpt {
+    const char *name;
+    std::vector<int64_t> values;
+};
+
+int autotune_run(mkde_ctx *c, const std::vector<TuneOpt> &opts, int reps, const std::function<int()> &work,
+                 std::string &rep, double *best_out)
+{
+    auto timed = [&](double *ms) -> int {
+        double best = 1e300;
+        for (int r = 0; r < reps; ++r) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            const auto t0 = std::chrono::steady_clock::now();
+            CHK(work());
+            HIPCHK(hipStreamSynchronize(c->stream));
+            best = std::min(best, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+        *ms = best;
+        return MKDE_OK;
+    };
+    CHK(work());   // warm-up (code objects, buffers at size)
+    double cur = 0.0;
+    CHK(timed(&cur));
+    char buf[256];
+    snprintf(buf, sizeof buf, "\"baseline_ms\": %.4f, \"trials\": [", cur);
+    rep += buf;
+    bool first = true;
+    for (const TuneOpt &o : opts) {
+        int64_t keep = 0;
+        CHK(mkde_get_option(c, o.name, &keep));
+        int64_t best_v = keep;
+        double best_ms = cur;
+        for (int64_t v : o.values) {
+            if (v == keep) continue;
+            CHK(mkde_set_option(c, o.name, v));
+            if (!strcmp(o.name, "points_per_bin")) CHK(mkde_points_upload(c, c->n, c->hx.data(), c->hy.data(), c->hw.data()));
+            double ms = 0.0;
+            CHK(timed(&ms));
+            snprintf(buf, sizeof buf, "%s{\"option\": \"%s\", \"value\": %lld, \"ms\": %.4f}", first ? "" : ", ", o.name,
+                     (long long)v, ms);
+            rep += buf;
+            first = false;
+            if (ms < best_ms * 0.99) {
+                best_ms = ms;
+                best_v = v;
+            }
+        }
+        CHK(mkde_set_option(c, o.name, best_v));
+        if (!strcmp(o.name, "points_per_bin")) CHK(mkde_points_upload(c, c->n, c->hx.data(), c->hy.data(), c->hw.data()));
+        cur = best_ms;
+    }
+    rep += "], \"chosen\": {";
+    first = true;
+    for (const TuneOpt &o : opts) {
+        int64_t v = 0;
+        CHK(mkde_get_option(c, o.name, &v));
+        snprintf(buf, sizeof buf, "%s\"%s\": %lld", first ? "" : ", ", o.name, (long long)v);
+        rep += buf;
+        first = false;
+    }
+    snprintf(buf, sizeof buf, "}, \"final_ms\": %.4f", cur);
+    rep += buf;
+    *best_out = cur;
+    return MKDE_OK;
+}
+
+int tune_report(const std::string &rep, char *report, int64_t cap)
+{
+    if (!report || cap <= 0) return MKDE_OK;
+    if ((int64_t)rep.size() + 1 > cap) return fail(MKDE_E_RANGE, "report needs %zu bytes", rep.size() + 1);
+    memcpy(report, rep.c_str(), rep.size() + 1);
+    return MKDE_OK;
+}
+}  // namespace
+
+extern "C" int mkde_autotune_job(mkde_ctx *c, const mkde_job_opts *o, int reps, char *report, int64_t report_cap)
+{
+    CHK(need_points(c));
+    if (!o || reps < 1 || reps > 100) return fail(MKDE_E_ARG, "mkde_autotune_job: bad arguments");
+    if (!c->tile_set) return fail(MKDE_E_STATE, "no density tile uploaded");
+    const bool ec = o->edge_correction != 0;
+    const bool loo = o->hoption == 1 || (o->hoption == 2 && o->optimise);
+    std::vector<TuneOpt> opts;
+    opts.push_back({"density_split_target", {2048, 4096, 8192, 16384}});
+    if (o->hoption == 2) opts.push_back({"pts_split_target", {4096, 16384, 65536}});
+    if (loo) opts.push_back({"loo_split_target", {8192, 32768, 131072}});
+    if (ec) {
+        opts.push_back({"ec_chunk", {512, 1024, 2048}});
+        opts.push_back({"ec_wide_min", {512, 1024, 2048, 4096}});
+    }
+    if (o->hoption != -1) opts.push_back({"points_per_bin", {16, 32, 64}});
+    std::string rep = "{\"workload\": \"job\", ";
+    double ms = 0.0;
+    CHK(autotune_run(c, opts, reps, [&]() { return mkde_run_job(c, o, nullptr); }, rep, &ms));
+    rep += "}";
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_table[tune_class(c->n, c->tg.ncols * c->tg.nrows, o->hoption, ec, o->optimise)] =
+            TuneChoice{c->tune, c->pts_per_bin};
+    }
+    return tune_report(rep, report, report_cap);
+}
+
+extern "C" int mkde_autotune_search(mkde_ctx *c, int K, const float *h_cand, int ec_on, int reps, char *report,
+                                    int64_t report_cap)
+{
+    CHK(need_points(c));
+    if (K <= 0 || K > 1024 || !h_cand || reps < 1 || reps > 100) return fail(MKDE_E_ARG, "mkde_autotune_search: bad arguments");
+    std::vector<double> L(K);
+    std::vector<TuneOpt> opts;
+    opts.push_back({"loo_split_target", {8192, 32768, 131072}});
+    if (ec_on) {
+        opts.push_back({"ec_chunk", {512, 1024, 2048}});
+        opts.push_back({"ec_wide_min", {512, 1024, 2048, 4096}});
+    }
+    opts.push_back({"points_per_bin", {16, 32, 64}});
+    std::string rep = "{\"workload\": \"search\", ";
+    double ms = 0.0;
+    CHK(autotune_run(c, opts, reps, [&]() { return mkde_loo_loglik(c, K, h_cand, ec_on, L.data()); }, rep, &ms));
+    rep += "}";
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_table[tune_class(c->n, c->mask_set ? c->mg.ncols * c->mg.nrows : 0, MKDE_H_SEARCH, ec_on, 0)] =
+            TuneChoice{c->tune, c->pts_per_bin};
+    }
+    return tune_report(rep, report, report_cap);
+}
+
+extern "C" int mkde_autotune_lookup(mkde_ctx *c, int64_t n, int64_t cells, int hoption, int ec, int optimise)
+{
+    if (!c) return fail(MKDE_E_ARG, "null context");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_table.find(tune_class(n, cells, hoption, ec, optimise));
+    if (it == g_tune_table.end()) return fail(MKDE_E_STATE, "no tuned settings for this job class");
+    c->tune = it->second.t;
+    c->pts_per_bin = it->second.ppb;
     return MKDE_OK;
 }
 

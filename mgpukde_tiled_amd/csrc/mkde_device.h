@@ -12,6 +12,36 @@
 // candidate bandwidths per batched leave-one-out pass
 #define MKDE_LM_K 16
 
+// Launch-shape settings of one context: defaults from the environment
+// (default_tuning), changed by mkde_set_option or the autotuner.  They set how
+// work is cut into pieces, so a change can move FP32 rounding (never the
+// cut-off sets or the error bound).
+struct Tuning {
+    int ec_chunk;                    // 512, 1024 or 2048: columns per prefix chunk of k_ec
+    int ec_wide_min;                 // boxes wider than this run in k_ec_wide
+    unsigned long long loo_target;   // split target of the batched leave-one-out work list
+    unsigned long long pts_target;   // split target of the point-density work list
+    unsigned long long dt_target;    // split target of the density-raster work list
+};
+
+// Smallest split threshold of the point work lists (candidate sources per part).
+#define MKDE_PTS_TMIN 4096ull
+
+// Most items and split parts a point work list of nw 64-query waves can have:
+// each wave's cost is at most n sources, the threshold is T = max(tmin,
+// total / target), so split parts <= min(target, total / tmin) + the number
+// of split waves.
+inline void point_work_bounds(int64_t n, unsigned long long target, int64_t *items_max, int64_t *slots_max)
+{
+    const int64_t nw = (n + 63) / 64;
+    const double tot = (double)nw * (double)n;
+    const int64_t by_t = (int64_t)((tot + MKDE_PTS_TMIN - 1) / (double)MKDE_PTS_TMIN);
+    const int64_t parts = by_t < (int64_t)target ? by_t : (int64_t)target;
+    const int64_t split_waves = by_t < nw ? by_t : nw;
+    *items_max = nw + parts + 1;
+    *slots_max = parts + split_waves + 1;
+}
+
 // Raster geometry (AsciiRaster.h:9-33 geometry fields).
 struct GridDesc {
     int64_t ncols, nrows;

@@ -944,8 +944,10 @@ __global__ void k_items_fill(int n, const unsigned *parts, const unsigned *item0
 // Heaviest-first order of the items: bucket counts and slots are reserved
 // once per workgroup (LDS counters, one global atomic per bucket), so the
 // few large buckets do not serialise every item on one address.
-__global__ __launch_bounds__(256) void k_item_hist(int n, const int4 *items, const unsigned *cost, unsigned *hist)
+__global__ __launch_bounds__(256) void k_item_hist(int n, const unsigned *n_dev, const int4 *items, const unsigned *cost,
+                                                   unsigned *hist)
 {
+    if (n_dev) n = (int)*n_dev;   // the item count on the device (grid sized for the most items)
     __shared__ unsigned s_h[32];
     if (threadIdx.x < 32) s_h[threadIdx.x] = 0;
     __syncthreads();
@@ -958,9 +960,11 @@ __global__ __launch_bounds__(256) void k_item_hist(int n, const int4 *items, con
     if (threadIdx.x < 32 && s_h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_h[threadIdx.x]);
 }
 
-__global__ __launch_bounds__(256) void k_item_order(int n, const int4 *items, const unsigned *cost, unsigned *offs,
+__global__ __launch_bounds__(256) void k_item_order(int n, const unsigned *n_dev, const int4 *items, const unsigned *cost,
+                                                    unsigned *offs,
                                                     int *order)
 {
+    if (n_dev) n = (int)*n_dev;
     __shared__ unsigned s_h[32], s_base[32];
     if (threadIdx.x < 32) s_h[threadIdx.x] = 0;
     __syncthreads();
@@ -1292,7 +1296,37 @@ struct LooMultiArgs {
     const unsigned *cost;
     double *partials;        // 64 * MKDE_LM_K per slot
     int n_items;
+    const unsigned *n_items_dev;   // non-null: the item count on the device (n_items: grid bound)
 };
+
+// Slot K of a source's candidate walk (then K-1, ... while a lane is inside):
+// acc[K] += (in ? exp2(d2f a_K) : 0) * coef_K, with slot K-1's mask made
+// before slot K's term.  While every lane is inside (the largest candidates
+// of a source that is near the whole wave) the term needs no select; the
+// select of a lane inside adds the same value, so both forms give the same
+// sums.  One wait state after the transcendental before its result is read;
+// a mask is at least two instructions old when v_cndmask reads it.
+template <int K>
+__device__ __forceinline__ void loo_step(uint64_t in, double d2, float d2f, const double (&thr)[LM_K],
+                                         const float (&af)[LM_K], const float (&cf)[LM_K], float (&acc)[LM_K])
+{
+    if (in == 0) return;
+    uint64_t next = 0;
+    if constexpr (K > 0) asm("v_cmp_gt_f64_e64 %0, %1, %2" : "=s"(next) : "s"(thr[K - 1]), "v"(d2));
+    float t;
+    asm("v_mul_f32_e32 %0, %2, %3\n\t"
+        "v_exp_f32_e32 %0, %0\n\t"
+        "s_nop 0\n\t"
+        "s_cmp_eq_u64 %4, -1\n\t"
+        "s_cbranch_scc1 1f\n\t"
+        "v_cndmask_b32_e64 %0, 0, %0, %4\n"
+        "1:\n\t"
+        "v_fmac_f32_e32 %1, %5, %0"
+        : "=&v"(t), "+v"(acc[K])
+        : "v"(d2f), "v"(af[K]), "s"(in), "s"(cf[K])
+        : "scc");
+    if constexpr (K > 0) loo_step<K - 1>(next, d2, d2f, thr, af, cf, acc);
+}
 
 // One source of the batched leave-one-out pass for the wave's 64 queries:
 // candidate slot k's term for a lane is exp2(d2f a_k) coef_k if the exact
@@ -1317,33 +1351,19 @@ __device__ __forceinline__ void loo_source(const PtRec *pts, const float *coefk,
     const float dx = p.x - qx, dy = p.y - qy;
     float d2f = __builtin_fmaf(dy, dy, dx * dx);
     const double d2 = sq2(dx, dy);
-    asm("v_cndmask_b32_e64 %0, %0, %1, %2" : "+v"(d2f) : "v"(inf), "s"(selfm));
-#pragma unroll
-    for (int k = LM_K - 1; k >= 0; --k) {
-        // the first compare also takes the source's coefficients, so their
-        // scalar loads are issued together and complete before the chain
-        uint64_t in;
-        if (k == LM_K - 1)
-            asm("v_cmp_gt_f64_e64 %0, %1, %2"
-                : "=s"(in)
-                : "s"(thr[k]), "v"(d2), "s"(cf[0]), "s"(cf[1]), "s"(cf[2]), "s"(cf[3]), "s"(cf[4]), "s"(cf[5]),
-                  "s"(cf[6]), "s"(cf[7]), "s"(cf[8]), "s"(cf[9]), "s"(cf[10]), "s"(cf[11]), "s"(cf[12]),
-                  "s"(cf[13]), "s"(cf[14]), "s"(cf[15]));
-        else
-            asm("v_cmp_gt_f64_e64 %0, %1, %2" : "=s"(in) : "s"(thr[k]), "v"(d2));
-        if (in == 0) break;
-        // acc += (in ? exp2(d2f a_k) : 0) * coef_k; one wait state after the
-        // transcendental before its result is read, and the mask is at least
-        // two instructions old when v_cndmask reads it
-        float t;
-        asm("v_mul_f32_e32 %0, %2, %3\n\t"
-            "v_exp_f32_e32 %0, %0\n\t"
-            "s_nop 0\n\t"
-            "v_cndmask_b32_e64 %0, 0, %0, %4\n\t"
-            "v_fmac_f32_e32 %1, %5, %0"
-            : "=&v"(t), "+v"(acc[k])
-            : "v"(d2f), "v"(af[k]), "s"(in), "s"(cf[k]));
-    }
+    if (selfm) asm("v_cndmask_b32_e64 %0, %0, %1, %2" : "+v"(d2f) : "v"(inf), "s"(selfm));
+    // slot k's mask is made one step ahead of its use: candidate k-1's compare
+    // issues before candidate k's term, so the scalar test of a mask never
+    // waits on the compare that made it.  The first compare also takes the
+    // source's coefficients, so their scalar loads are issued together and
+    // complete before the chain.
+    uint64_t in;
+    asm("v_cmp_gt_f64_e64 %0, %1, %2"
+        : "=s"(in)
+        : "s"(thr[LM_K - 1]), "v"(d2), "s"(cf[0]), "s"(cf[1]), "s"(cf[2]), "s"(cf[3]), "s"(cf[4]), "s"(cf[5]),
+          "s"(cf[6]), "s"(cf[7]), "s"(cf[8]), "s"(cf[9]), "s"(cf[10]), "s"(cf[11]), "s"(cf[12]), "s"(cf[13]),
+          "s"(cf[14]), "s"(cf[15]));
+    loo_step<LM_K - 1>(in, d2, d2f, thr, af, cf, acc);
 }
 
 // Batched leave-one-out densities of up to 16 fixed bandwidths in one pass
@@ -1364,7 +1384,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
     const int off = LM_K - A.K;
     const int64_t n = I.n;
     const int64_t g = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (g >= (A.order ? (int64_t)A.n_items : (n + 63) / 64)) return;
+    const int64_t units = A.n_items_dev ? (int64_t)*A.n_items_dev : A.order ? (int64_t)A.n_items : (n + 63) / 64;
+    if (g >= units) return;
     const int4 item = uniform_item(A.order ? A.items[A.order[g]] : make_int4((int)g, 0, 1, -1));
     int win0 = 0, win1 = INT_MAX;   // candidate counts are below 2^31
     if (item.z > 1) {
@@ -1449,8 +1470,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
 
 // Parts of split waves: FP64 partials added in part order, rounded once.
 // splits[k] = {wave, slot0, parts}; m = values per query (1, or MKDE_LM_K).
-__global__ void k_point_merge(int64_t n, int m, int K, const int4 *splits, const double *partials, float *den)
+__global__ void k_point_merge(int64_t n, int m, int K, const unsigned *n_split_dev, const int4 *splits,
+                              const double *partials, float *den)
 {
+    if (n_split_dev && blockIdx.x >= *n_split_dev) return;   // grid sized for the most splits
     const int4 sp = splits[blockIdx.x];
     const int lane = threadIdx.x & 63, kk = threadIdx.x >> 6;   // 64 x 4 threads
     const int64_t q = (int64_t)sp.x * 64 + lane;
@@ -1798,12 +1821,19 @@ struct EcArgs {
     MaskRuns runs;
     const float *colx, *rowy;   // cell-centre tables of the mask grid
     const float *x, *y;     // original order
-    const float *h;         // original order (nullable when h_fixed > 0)
+    const float *h;         // original order (nullable when h_fixed > 0 or h_dev)
     float h_fixed;
+    const float *h_dev;     // non-null: every point's bandwidth is *h_dev (a device value)
+    const int *count;       // non-null: {narrow, wide} list lengths on the device
     const int *list;        // active points
     float *ew;              // original order
     unsigned long long *n_eval;
 };
+
+__device__ __forceinline__ float ec_bandwidth(const EcArgs &A, int p)
+{
+    return A.h_dev ? *A.h_dev : (A.h_fixed > 0.0f ? A.h_fixed : A.h[p]);
+}
 
 // NARROW box with the reference's macro semantics (Utilities.h:16-17,30-31;
 // tiled:2716-2727): size_t arithmetic for row_upper / col_upper.
@@ -1888,8 +1918,9 @@ __device__ __forceinline__ void row_interval(const COL &colx, double xll, double
 // count[1] wide points from the back (wider boxes up to cw_wide columns:
 // k_ec_wide, the whole box in LDS), count[2] the widest wide box.
 __global__ __launch_bounds__(256) void k_ec_active(int64_t n, GridDesc g, const float *x, const float *y,
-                                                   const double *dist, const float *h, float h_fixed, float *ew,
-                                                   int *list, int *count, int cw_narrow, int cw_wide)
+                                                   const double *dist, const float *h, float h_fixed,
+                                                   const float *h_dev, float *ew, int *list, int *count,
+                                                   int cw_narrow, int cw_wide)
 {
     // list slots are reserved per workgroup (LDS counters, then one global
     // atomic per list): per-wave atomics on one address serialise
@@ -1903,7 +1934,7 @@ __global__ __launch_bounds__(256) void k_ec_active(int64_t n, GridDesc g, const 
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
     int kind = -1, slot = 0;   // 0 narrow, 1 wide
     if (i < n) {
-        const float hh = h_fixed > 0.0f ? h_fixed : h[i];
+        const float hh = h_dev ? *h_dev : (h_fixed > 0.0f ? h_fixed : h[i]);
         const double h2 = (double)(float)(hh * hh);
         if (dist[i] >= 9.0 * h2) {
             ew[i] = 1.0f;
@@ -1975,17 +2006,16 @@ __device__ __forceinline__ void ec_row(const EcArgs &A, const double *s_pd, int 
     acc = __builtin_fma(seg, (double)ey, acc);
 }
 
+// EC weight of point p by one wave: column chunks of EC_CW with the FP64
+// prefix sums of Ex in s_pd, rows on lanes.
 template <int EC_CW>
-__global__ __launch_bounds__(64) void k_ec(EcArgs A)
+__device__ __forceinline__ void ec_point(const EcArgs &A, double *s_pd, int p)
 {
-    __shared__ double s_pd[EC_CW];   // prefix sums of Ex over the chunk's columns
-
     const int lane = threadIdx.x;
-    const int p = A.list[blockIdx.x];
     const GridDesc g = A.g;
     const double xll = g.xll, cs = g.cs, inv_cs = 1.0 / cs;
     const float px = A.x[p], py = A.y[p];
-    const float hh = A.h_fixed > 0.0f ? A.h_fixed : A.h[p];
+    const float hh = ec_bandwidth(A, p);
     const double h2 = (double)(float)(hh * hh);
     const double thr = 9.0 * h2;
     const double a = -MKDE_LOG2E / (2.0 * h2);
@@ -2045,24 +2075,30 @@ __global__ __launch_bounds__(64) void k_ec(EcArgs A)
     }
 }
 
+// One wave per point of the narrow part of the list: a workgroup takes list
+// items blockIdx.x, blockIdx.x + gridDim.x, ... below the narrow count (the
+// device count when given, else one item per workgroup).
+template <int EC_CW>
+__global__ __launch_bounds__(64) void k_ec(EcArgs A)
+{
+    __shared__ double s_pd[EC_CW];   // prefix sums of Ex over the chunk's columns
+    const int lim = A.count ? A.count[0] : (int)gridDim.x;
+    for (int it = blockIdx.x; it < lim; it += gridDim.x) ec_point<EC_CW>(A, s_pd, A.list[it]);
+}
+
 // Wide points (NARROW box wider than the column chunk of k_ec): one
 // 256-thread workgroup per point and the whole box's prefix sums in dynamic
 // LDS (8 B per column), so every row is visited once instead of once per
 // chunk.  Same row arithmetic as k_ec; the prefix is scanned across the
 // workgroup (four waves).
-__global__ __launch_bounds__(256) void k_ec_wide(EcArgs A, int64_t n)
+__device__ __forceinline__ void ec_point_wide(const EcArgs &A, double *s_pd, double *s_wt, double *s_acc,
+                                              unsigned long long *s_cnt, int p)
 {
-    extern __shared__ double s_pd[];
-    __shared__ double s_wt[4];
-    __shared__ double s_acc[4];
-    __shared__ unsigned long long s_cnt[4];
-
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int p = A.list[n - 1 - blockIdx.x];
     const GridDesc g = A.g;
     const double xll = g.xll, cs = g.cs, inv_cs = 1.0 / cs;
     const float px = A.x[p], py = A.y[p];
-    const float hh = A.h_fixed > 0.0f ? A.h_fixed : A.h[p];
+    const float hh = ec_bandwidth(A, p);
     const double h2 = (double)(float)(hh * hh);
     const double thr = 9.0 * h2;
     const double a = -MKDE_LOG2E / (2.0 * h2);
@@ -2116,6 +2152,22 @@ __global__ __launch_bounds__(256) void k_ec_wide(EcArgs A, int64_t n)
         const double ec = tot * (double)(float)(g.cs * g.cs) / (h2 * (double)MKDE_TWO_PI_F);
         A.ew[p] = (float)(1.0 / ec);
         if (A.n_eval) slot_add(A.n_eval, s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3]);
+    }
+}
+
+// One workgroup per point of the wide part of the list (stored from the back
+// of the list), items blockIdx.x, blockIdx.x + gridDim.x, ... below the wide
+// count (the device count when given, else one item per workgroup).
+__global__ __launch_bounds__(256) void k_ec_wide(EcArgs A, int64_t n)
+{
+    extern __shared__ double s_pd[];
+    __shared__ double s_wt[4];
+    __shared__ double s_acc[4];
+    __shared__ unsigned long long s_cnt[4];
+    const int lim = A.count ? A.count[1] : (int)gridDim.x;
+    for (int it = blockIdx.x; it < lim; it += gridDim.x) {
+        ec_point_wide(A, s_pd, s_wt, s_acc, s_cnt, A.list[n - 1 - it]);
+        __syncthreads();   // the next point rewrites the LDS this one read
     }
 }
 
@@ -2261,6 +2313,18 @@ __global__ void k_coef_k(int64_t n, const int *perm, const float *w_orig, const 
     coef_out[i * stride] = (float)((double)w_orig[o] * (double)ew_k_orig[o] / (h2k * (double)MKDE_TWO_PI_F));
 }
 
+// k_coef_k with the bandwidth read from the device: h2k = fl32(h*h).
+__global__ void k_coef_k_dev(int64_t n, const int *perm, const float *w_orig, const float *ew_k_orig, const float *h_dev,
+                             float *coef_out, int stride)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float h = *h_dev;
+    const double h2k = (double)(float)(h * h);
+    const int o = perm[i];
+    coef_out[i * stride] = (float)((double)w_orig[o] * (double)ew_k_orig[o] / (h2k * (double)MKDE_TWO_PI_F));
+}
+
 __global__ void k_fill_f32(int64_t n, float v, float *dst)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2309,7 +2373,7 @@ static void scan_u32(hipStream_t s, int n, const unsigned *in, unsigned *out, un
 
 // Costs, split parts and the three scans; W.counts receives {items, slots, splits}.
 hipError_t density_plan(hipStream_t s, const GridDesc &g, const float *mask, const PointIndex &idx, const Tiling &tl,
-                        const DensityWork &W)
+                        const DensityWork &W, const Tuning &T)
 {
     const int nbx = (int)((g.ncols + MKDE_DT_BW - 1) / MKDE_DT_BW);
     const int nblk = (int)raster_blocks(g, tl);
@@ -2317,15 +2381,7 @@ hipError_t density_plan(hipStream_t s, const GridDesc &g, const float *mask, con
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, (int64_t)2, 0.0f, (float *)W.total);
     const unsigned nred = (unsigned)std::min<int64_t>(1024, ((int64_t)nblk + 255) / 256);
     hipLaunchKernelGGL(k_cost_total, dim3(nred), dim3(256), 0, s, nblk, W.cost, W.total);
-    // measured (adaptive 1M job / 10M-point tiles, density ms): target 1024:
-    // 30.9 / 32.7, 2048: 30.9 / 27.5, 4096: 31.0 / 27.8, 16384: 31.7 / 27.9,
-    // 65536: 33.3 / 29.2; MKDE_DENSITY_TARGET overrides the default 4096
-    static const unsigned long long dt_target = [] {
-        const char *e = getenv("MKDE_DENSITY_TARGET");
-        const long long v = e ? atoll(e) : 0;
-        return v > 0 ? (unsigned long long)v : DT_TARGET;
-    }();
-    hipLaunchKernelGGL(k_parts, dim3(nblk_(nblk)), dim3(256), 0, s, nblk, W.cost, W.total, DT_TMIN, dt_target, W.parts,
+    hipLaunchKernelGGL(k_parts, dim3(nblk_(nblk)), dim3(256), 0, s, nblk, W.cost, W.total, DT_TMIN, T.dt_target, W.parts,
                        W.item0, W.sidx);
     // in place is not allowed: parts -> item0 scan reads parts; sparts / sflag
     // were written to item0 / sidx and are scanned into slot0 / sidx2 below
@@ -2335,15 +2391,18 @@ hipError_t density_plan(hipStream_t s, const GridDesc &g, const float *mask, con
     return hipGetLastError();
 }
 
+// n_dev non-null: the item count is read on the device (W.counts[0]) and
+// n_items is only the grid bound (the most items the work list can have).
 hipError_t density_items(hipStream_t s, int nblk, const DensityWork &W, int n_items, int4 *items, int4 *splits,
-                         int *order)
+                         int *order, const unsigned *n_dev)
 {
     hipLaunchKernelGGL(k_items_fill, dim3(nblk_(nblk)), dim3(256), 0, s, nblk, W.parts, W.item0, W.slot0, W.sidx2,
                        items, splits);
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, (int64_t)32, 0.0f, (float *)W.hist);
-    hipLaunchKernelGGL(k_item_hist, dim3(nblk_(n_items)), dim3(256), 0, s, n_items, items, W.cost, W.hist);
+    hipLaunchKernelGGL(k_item_hist, dim3(nblk_(n_items)), dim3(256), 0, s, n_items, n_dev, items, W.cost, W.hist);
     hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(64), 0, s, W.hist);
-    hipLaunchKernelGGL(k_item_order, dim3(nblk_(n_items)), dim3(256), 0, s, n_items, items, W.cost, W.hist, order);
+    hipLaunchKernelGGL(k_item_order, dim3(nblk_(n_items)), dim3(256), 0, s, n_items, n_dev, items, W.cost, W.hist,
+                       order);
     return hipGetLastError();
 }
 
@@ -2375,7 +2434,7 @@ hipError_t density_merge(hipStream_t s, const GridDesc &g, const Tiling &tl, con
 // kind 0: density at points (h_fixed > 0: one bandwidth), 1: batched
 // leave-one-out with thresholds thr_k[0..K-1] ascending.
 hipError_t point_plan(hipStream_t s, const PointIndex &idx, int kind, float h_fixed, const double *thr_k, int K,
-                      const DensityWork &W)
+                      const DensityWork &W, const Tuning &T)
 {
     const int nw = (int)((idx.n + 63) / 64);
     double lim_fixed = 0.0;
@@ -2389,24 +2448,8 @@ hipError_t point_plan(hipStream_t s, const PointIndex &idx, int kind, float h_fi
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, (int64_t)2, 0.0f, (float *)W.total);
     const unsigned nred = (unsigned)std::min<int64_t>(1024, ((int64_t)nw + 255) / 256);
     hipLaunchKernelGGL(k_cost_total, dim3(nred), dim3(256), 0, s, nw, W.cost, W.total);
-    const unsigned tmin = 4096;
-    // split target of the 16-candidate leave-one-out pass: 32768 measured best
-    // on the 200k-point 64-candidate search (2048 / 8192 / 32768 / 65536 /
-    // 131072: 237 / 200 / 187 / 191 / 194 ms); MKDE_LOO_TARGET overrides it
-    static const unsigned long long target_loo = [] {
-        const char *e = getenv("MKDE_LOO_TARGET");
-        const long long v = e ? atoll(e) : 0;
-        return v > 0 ? (unsigned long long)v : 32768ull;
-    }();
-    // point densities (pilot): 16384 measured best on the adaptive 1M job
-    // (4096 / 16384 / 65536: bandwidth phase 8.2 / 7.5 / 8.0 ms);
-    // MKDE_PTS_TARGET overrides it
-    static const unsigned long long target_pts = [] {
-        const char *e = getenv("MKDE_PTS_TARGET");
-        const long long v = e ? atoll(e) : 0;
-        return v > 0 ? (unsigned long long)v : 16384ull;
-    }();
-    const unsigned long long target = kind == 1 ? target_loo : target_pts;
+    const unsigned tmin = (unsigned)MKDE_PTS_TMIN;
+    const unsigned long long target = kind == 1 ? T.loo_target : T.pts_target;
     hipLaunchKernelGGL(k_parts, dim3(nblk_(nw)), dim3(256), 0, s, nw, W.cost, W.total, tmin, target, W.parts,
                        W.item0, W.sidx);
     scan_u32(s, nw, W.item0, W.slot0, W.chunks, W.counts + 1);
@@ -2435,22 +2478,28 @@ hipError_t point_density(hipStream_t s, const PointIndex &idx, int loo, float *d
     return hipGetLastError();
 }
 
+// n_items_dev non-null: the item count is read on the device and n_items is
+// only the grid bound.
 hipError_t loo_multi(hipStream_t s, const PointIndex &idx, const float *coefk, const double *a_k,
                      const double *thr_k, int K, float *den1, const int *order, const int4 *items,
-                     const unsigned *cost, double *partials, int n_items)
+                     const unsigned *cost, double *partials, int n_items,
+                     const unsigned *n_items_dev)
 {
-    LooMultiArgs A{idx, coefk, a_k, thr_k, K, den1, order, items, cost, partials, n_items};
+    LooMultiArgs A{idx, coefk, a_k, thr_k, K, den1, order, items, cost, partials, n_items, n_items_dev};
     if (idx.n <= 0) return hipSuccess;
     const int64_t units = order ? n_items : (idx.n + 63) / 64;
     hipLaunchKernelGGL(k_loo_multi_w, dim3((unsigned)((units + 3) / 4)), dim3(256), 0, s, A);
     return hipGetLastError();
 }
 
+// n_split_dev non-null: the split count is read on the device and n_split is
+// only the grid bound.
 hipError_t point_merge(hipStream_t s, int64_t n, int m, int K, int n_split, const int4 *splits, const double *partials,
-                       float *den)
+                       float *den, const unsigned *n_split_dev)
 {
     if (n_split > 0)
-        hipLaunchKernelGGL(k_point_merge, dim3((unsigned)n_split), dim3(256), 0, s, n, m, K, splits, partials, den);
+        hipLaunchKernelGGL(k_point_merge, dim3((unsigned)n_split), dim3(256), 0, s, n, m, K, n_split_dev, splits,
+                           partials, den);
     return hipGetLastError();
 }
 
@@ -2509,12 +2558,14 @@ hipError_t dist_boundary(hipStream_t s, int64_t n, const float *x, const float *
     return hipGetLastError();
 }
 
+// h_dev non-null: every point's bandwidth is the device value *h_dev.
 hipError_t ec_active(hipStream_t s, int64_t n, const GridDesc &g, const float *x, const float *y, const double *dist,
-                     const float *h, float h_fixed, float *ew, int *list, int *count, int cw_narrow, int cw_wide)
+                     const float *h, float h_fixed, const float *h_dev, float *ew, int *list, int *count,
+                     int cw_narrow, int cw_wide)
 {
     if (n > 0)
-        hipLaunchKernelGGL(k_ec_active, dim3(nblk(n, 256)), dim3(256), 0, s, n, g, x, y, dist, h, h_fixed, ew, list,
-                           count, cw_narrow, cw_wide);
+        hipLaunchKernelGGL(k_ec_active, dim3(nblk(n, 256)), dim3(256), 0, s, n, g, x, y, dist, h, h_fixed, h_dev, ew,
+                           list, count, cw_narrow, cw_wide);
     return hipGetLastError();
 }
 
@@ -2525,32 +2576,39 @@ hipError_t centres(hipStream_t s, const GridDesc &g, float *colx, float *rowy)
     return hipGetLastError();
 }
 
-// Column chunk of k_ec's prefix sums (LDS 8 B per column and wave); 1024
-// measured best (512 / 1024 / 2048: the adaptive job's EC within 1%, the
-// 64-candidate search 314 / 296 / 307 ms); MKDE_EC_CHUNK overrides it.
-int ec_chunk()
-{
-    static const int cw = [] {
-        const char *e = getenv("MKDE_EC_CHUNK");
-        const int v = e ? atoi(e) : 1024;
-        return (v == 512 || v == 2048) ? v : 1024;
-    }();
-    return cw;
-}
-
 // Widest box k_ec_wide takes: its prefix sums fill at most 64,000 B of LDS.
 int ec_wide_max() { return 8000; }
 
-// Boxes wider than this go to k_ec_wide (default: the chunk width, i.e. every
-// box k_ec would cut into two or more chunks); MKDE_EC_WIDE_MIN overrides it.
-int ec_wide_min()
+static long long env_ll(const char *name, long long dflt)
 {
-    static const int v = [] {
-        const char *e = getenv("MKDE_EC_WIDE_MIN");
-        const int x = e ? atoi(e) : 0;
-        return x > 0 ? x : 0;
-    }();
-    return v > 0 ? v : ec_chunk();
+    const char *e = getenv(name);
+    const long long v = e ? atoll(e) : 0;
+    return v > 0 ? v : dflt;
+}
+
+// Default launch shapes (measured on one MI355X), each overridable from the
+// environment:
+//   ec_chunk 1024 (MKDE_EC_CHUNK; 512 / 1024 / 2048: the adaptive job's EC
+//     within 1%, the 64-candidate search 314 / 296 / 307 ms);
+//   ec_wide_min = ec_chunk (MKDE_EC_WIDE_MIN): every box k_ec would cut into
+//     two or more chunks runs in k_ec_wide;
+//   loo_target 32768 (MKDE_LOO_TARGET; 2048 / 8192 / 32768 / 65536 / 131072:
+//     search 237 / 200 / 187 / 191 / 194 ms);
+//   pts_target 16384 (MKDE_PTS_TARGET; 4096 / 16384 / 65536: bandwidth phase
+//     of the adaptive 1M job 8.2 / 7.5 / 8.0 ms);
+//   dt_target 4096 (MKDE_DENSITY_TARGET; adaptive 1M job / 10M-point tiles,
+//     density ms: 1024: 30.9 / 32.7, 2048: 30.9 / 27.5, 4096: 31.0 / 27.8,
+//     16384: 31.7 / 27.9, 65536: 33.3 / 29.2).
+Tuning default_tuning()
+{
+    Tuning t;
+    const long long cw = env_ll("MKDE_EC_CHUNK", 1024);
+    t.ec_chunk = (cw == 512 || cw == 2048) ? (int)cw : 1024;
+    t.ec_wide_min = (int)env_ll("MKDE_EC_WIDE_MIN", t.ec_chunk);
+    t.loo_target = (unsigned long long)env_ll("MKDE_LOO_TARGET", 32768);
+    t.pts_target = (unsigned long long)env_ll("MKDE_PTS_TARGET", 16384);
+    t.dt_target = (unsigned long long)env_ll("MKDE_DENSITY_TARGET", (long long)DT_TARGET);
+    return t;
 }
 
 // The n_narrow points at the front of list run in k_ec (one wave each, column
@@ -2558,10 +2616,9 @@ int ec_wide_min()
 // max_w columns of LDS).
 hipError_t ec(hipStream_t s, int64_t n, int n_narrow, int n_wide, int max_w, const GridDesc &g, const MaskRuns &runs,
               const float *colx, const float *rowy, const float *x, const float *y, const float *h, float h_fixed,
-              const int *list, float *ew, unsigned long long *n_eval)
+              const int *list, float *ew, unsigned long long *n_eval, int cw)
 {
-    EcArgs A{g, runs, colx, rowy, x, y, h, h_fixed, list, ew, n_eval};
-    const int cw = ec_chunk();
+    EcArgs A{g, runs, colx, rowy, x, y, h, h_fixed, nullptr, nullptr, list, ew, n_eval};
     if (n_narrow > 0) {
         if (cw == 512) hipLaunchKernelGGL(k_ec<512>, dim3((unsigned)n_narrow), dim3(64), 0, s, A);
         else if (cw == 1024) hipLaunchKernelGGL(k_ec<1024>, dim3((unsigned)n_narrow), dim3(64), 0, s, A);
@@ -2571,6 +2628,26 @@ hipError_t ec(hipStream_t s, int64_t n, int n_narrow, int n_wide, int max_w, con
         if (max_w < 1 || max_w > ec_wide_max()) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_ec_wide, dim3((unsigned)n_wide), dim3(256), (size_t)max_w * sizeof(double), s, A, n);
     }
+    return hipGetLastError();
+}
+
+// The same EC pass with every size on the device (no host read-back, so the
+// launches can be captured in a graph): the bandwidth is *h_dev, the list
+// lengths count[0] (narrow) and count[1] (wide) as k_ec_active left them.
+// Fixed grids loop over the list; the wide kernel always takes the largest
+// box's LDS (ec_wide_max columns).
+hipError_t ec_dev(hipStream_t s, int64_t n, const GridDesc &g, const MaskRuns &runs, const float *colx,
+                  const float *rowy, const float *x, const float *y, const float *h_dev, const int *count,
+                  const int *list, float *ew, int cw)
+{
+    if (n <= 0) return hipSuccess;
+    EcArgs A{g, runs, colx, rowy, x, y, nullptr, 0.0f, h_dev, count, list, ew, nullptr};
+    const unsigned gn = (unsigned)std::min<int64_t>(n, 256 * 32);   // 8 waves per SIMD
+    const unsigned gw = (unsigned)std::min<int64_t>(n, 256 * 2);    // 2 workgroups per CU (64,000 B of LDS)
+    if (cw == 512) hipLaunchKernelGGL(k_ec<512>, dim3(gn), dim3(64), 0, s, A);
+    else if (cw == 1024) hipLaunchKernelGGL(k_ec<1024>, dim3(gn), dim3(64), 0, s, A);
+    else hipLaunchKernelGGL(k_ec<2048>, dim3(gn), dim3(64), 0, s, A);
+    hipLaunchKernelGGL(k_ec_wide, dim3(gw), dim3(256), (size_t)ec_wide_max() * sizeof(double), s, A, n);
     return hipGetLastError();
 }
 
@@ -2647,6 +2724,14 @@ hipError_t coef_k(hipStream_t s, int64_t n, const int *perm, const float *w, con
 {
     if (n > 0)
         hipLaunchKernelGGL(k_coef_k, dim3(nblk(n, 256)), dim3(256), 0, s, n, perm, w, ew_k, h2k, coef_out, stride);
+    return hipGetLastError();
+}
+
+hipError_t coef_k_dev(hipStream_t s, int64_t n, const int *perm, const float *w, const float *ew_k, const float *h_dev,
+                      float *coef_out, int stride)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_coef_k_dev, dim3(nblk(n, 256)), dim3(256), 0, s, n, perm, w, ew_k, h_dev, coef_out, stride);
     return hipGetLastError();
 }
 

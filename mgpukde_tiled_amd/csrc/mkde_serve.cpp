@@ -40,6 +40,9 @@
 #include "mkde_device.h"
 
 extern "C" int mkde_internal_fail(int code, const char *msg);
+namespace mkde_dev {
+Tuning default_tuning();
+}
 
 namespace {
 
@@ -123,18 +126,23 @@ extern "C" int64_t mkde_job_footprint(int64_t n, int64_t ncols, int64_t nrows, i
     if (loo) b += 148 * n;
     // mask, valid flags, boundary flags and list, row runs, density output
     b += 24 * ncols * nrows + 16 * nrows;
-    // FP64 partials of split 64-point waves of the point kernels: at most
-    // 2 * cost / 4096 parts per wave (cost <= n) and at most twice the split
-    // target: 2 * 16384 (density at points, 512 B each) or 2 * 32768
-    // (16-candidate pass, 8 KB each; MKDE_LOO_TARGET is not taken into account)
     // FP64 partials of split density blocks (8 B per cell): at most twice the
-    // split target (2 * 4096) and 2 * cost / 2048 + 1 per block (cost <= n)
+    // split target and 2 * cost / 2048 + 1 per block (cost <= n)
+    const Tuning tune = mkde_dev::default_tuning();
     const double nblk = (double)((ncols + MKDE_DT_BW - 1) / MKDE_DT_BW) * (double)((nrows + 63) / 64);
-    b += (int64_t)std::min(8192.0, 2.0 * (double)n * nblk / 2048.0 + nblk) * (8 * MKDE_DT_CELLS);
-    const int64_t nw = (n + 63) / 64;
-    const double parts = 2.0 * (double)nw * (double)n / 4096.0;
-    b += (int64_t)std::min(parts, 32768.0) * 512;
-    if (loo) b += (int64_t)std::min(parts, 65536.0) * 8192;
+    b += (int64_t)std::min(2.0 * (double)tune.dt_target, 2.0 * (double)n * nblk / 2048.0 + nblk) *
+         (8 * MKDE_DT_CELLS);
+    // work lists of the point kernels at their largest (point_work_bounds):
+    // items (20 B each) and FP64 partials of split 64-query waves, 512 B per
+    // part for densities at points and 8 KB per part for a 16-candidate pass
+    // (a bandwidth search sizes them so for its captured graph)
+    int64_t items = 0, slots = 0;
+    point_work_bounds(n, tune.pts_target, &items, &slots);
+    b += items * 20 + slots * 512;
+    if (loo) {
+        point_work_bounds(n, tune.loo_target, &items, &slots);
+        b += items * 20 + slots * 8192;
+    }
     return b;
 }
 
