@@ -138,6 +138,18 @@ int mkde_range_candidates(mkde_ctx *ctx, int64_t *offsets, int32_t *idx, int64_t
  * loo = 0: den0 (self term included); loo = 1: den1 (self term excluded). */
 int mkde_density_at_points(mkde_ctx *ctx, int loo, float *den);
 
+/* Allocation-free form of mkde_density_at_points, with no counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and result; den_dev is the caller's DEVICE buffer of n floats
+ * (caller's point order).  The work list, the split partial sums and the
+ * bin-order densities live in a static 32 MiB array per device (held until the
+ * process ends) instead of context buffers; no hipMalloc / hipFree is called.
+ * About 5 million points fit; more return MKDE_E_RANGE before any kernel runs.
+ * Every call on one device shares that array: calls that overlap on one GPU
+ * (two contexts, two threads) race and must be serialised by the caller. */
+int mkde_density_at_points_noalloc(mkde_ctx *ctx, int loo, float *den_dev);
+
 /* Log-likelihood of K fixed bandwidths (LogLikelihood fixed-h,
  * tiled:3214-3368), all candidates in one batched pass over point pairs.
  * ec_on != 0 recomputes EC weights per candidate (UPDATEWEIGHTS).  logL[k] is

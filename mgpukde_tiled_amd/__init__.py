@@ -177,6 +177,7 @@ SIGNATURES = {
     "mkde_get_ec_weights": [_CTX, _FP],
     "mkde_range_candidates": [_CTX, _I64P, _I32P, ctypes.c_int64, _I64P],
     "mkde_density_at_points": [_CTX, ctypes.c_int, _FP],
+    "mkde_density_at_points_noalloc": [_CTX, ctypes.c_int, _FP],
     "mkde_loo_loglik": [_CTX, ctypes.c_int, _FP, ctypes.c_int, _DP],
     "mkde_loo_loglik_adaptive": [_CTX, ctypes.c_int, _FP, _FP, ctypes.c_int, _DP],
     "mkde_abramson": [_CTX, ctypes.c_float, ctypes.c_float, ctypes.c_int, _FP, _FP, _DP],
@@ -380,6 +381,11 @@ class Context:
         out = np.empty(self.n, np.float32)
         _check(_lib.mkde_density_at_points(self._h, 1 if loo else 0, _p(out, _FP)))
         return out
+
+    def density_at_points_noalloc(self, loo, den_dev):
+        """The allocation-free form: den_dev is a device address (int) of n
+        floats the caller owns; nothing is allocated on the device."""
+        _check(_lib.mkde_density_at_points_noalloc(self._h, 1 if loo else 0, ctypes.cast(den_dev, _FP)))
 
     def LogLikelihood(self, h_candidates, edge_correction=True):
         """FP64 LOO log-likelihood of each fixed bandwidth (one batched pass)."""

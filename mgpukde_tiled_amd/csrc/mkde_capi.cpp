@@ -76,6 +76,7 @@ hipError_t sort_f32(hipStream_t, int64_t, const int *, const float *, float *);
 hipError_t coef_k(hipStream_t, int64_t, const int *, const float *, const float *, double, float *, int);
 hipError_t coef_k_dev(hipStream_t, int64_t, const int *, const float *, const float *, const float *, float *, int);
 hipError_t fill_f32(hipStream_t, int64_t, float, float *);
+hipError_t scratch_address(int, void **);
 }  // namespace mkde_dev
 
 // ------------------------------------------------------------ errors ----
@@ -156,6 +157,8 @@ struct mkde_ctx {
     std::atomic<int64_t> dev_bytes{0};   // device memory held by the buffers below
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // the device's static scratch array (allocation-free entry point)
+    unsigned char *scratch = nullptr;
     // events of a job's phase timers, created once and reused by every job
     std::vector<hipEvent_t> ev_pool;
     size_t ev_next = 0;
@@ -292,6 +295,9 @@ extern "C" int mkde_ctx_create(int device, mkde_ctx **out)
     const char *sg = getenv("MKDE_SEARCH_GRAPH");
     if (sg && sg[0] == '0') c->search_graph_on = false;
     e = hipSetDevice(device);
+    // the static scratch array's address on this device, resolved once per
+    // device with the device current
+    if (e == hipSuccess) e = mkde_dev::scratch_address(device, (void **)&c->scratch);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
@@ -334,6 +340,7 @@ extern "C" int mkde_ctx_release(mkde_ctx *c)
     const int dev = c->device;
     hipStream_t s = c->stream;
     hipEvent_t e0 = c->ev0, e1 = c->ev1;
+    unsigned char *scr = c->scratch;
     std::vector<hipEvent_t> pool;
     pool.swap(c->ev_pool);
     const bool order_on = c->dt_order_on, graph_on = c->search_graph_on;
@@ -347,6 +354,7 @@ extern "C" int mkde_ctx_release(mkde_ctx *c)
     c->stream = s;
     c->ev0 = e0;
     c->ev1 = e1;
+    c->scratch = scr;
     c->ev_pool.swap(pool);
     c->dt_order_on = order_on;
     c->search_graph_on = graph_on;
@@ -1011,6 +1019,75 @@ extern "C" int mkde_density_at_points(mkde_ctx *c, int loo, float *den)
     if (den) CHK(copy_d2h(c, den, c->den2.p, c->n * sizeof(float)));
     HIPCHK(hipEventSynchronize(c->ev1));
     HIPCHK(hipEventElapsedTime(&c->last_points_ms, c->ev0, c->ev1));
+    return MKDE_OK;
+}
+
+// Allocation-free variant of mkde_density_at_points: the same kernels, with
+// the work list, the FP64 partial sums of split waves and the bin-order
+// densities carved from the device's static scratch array (MKDE_SCRATCH_BYTES)
+// instead of context buffers, and the result written to the caller's device
+// buffer.  Calls no hipMalloc / hipFree.  Sizes beyond the scratch return
+// MKDE_E_RANGE before any kernel runs.
+namespace {
+struct Carve {
+    unsigned char *base;
+    size_t cap, used = 0;
+    bool ok = true;
+    Carve(unsigned char *b, size_t c) : base(b), cap(c) {}
+    template <class T>
+    T *take(size_t count)
+    {
+        const size_t off = (used + 255) & ~(size_t)255;   // 256-byte aligned pieces
+        const size_t bytes = (count ? count : 1) * sizeof(T);
+        if (!ok || off > cap || bytes > cap - off) {
+            ok = false;
+            used = off + bytes;
+            return nullptr;
+        }
+        used = off + bytes;
+        return (T *)(base + off);
+    }
+};
+}  // namespace
+
+extern "C" int mkde_density_at_points_noalloc(mkde_ctx *c, int loo, float *den_dev)
+{
+    CHK(need_points(c));
+    CHK(need_h(c));
+    if (!den_dev) return fail(MKDE_E_ARG, "den_dev is NULL");
+    if (!c->scratch) return fail(MKDE_E_STATE, "the scratch array of device %d is not resolved", c->device);
+    const int64_t n = c->n, nw = (n + 63) / 64;
+    Carve cv(c->scratch, MKDE_SCRATCH_BYTES);
+    float *den_s = cv.take<float>((size_t)n);
+    unsigned *cost = cv.take<unsigned>(nw), *parts = cv.take<unsigned>(nw), *item0 = cv.take<unsigned>(nw);
+    unsigned *slot0 = cv.take<unsigned>(nw), *sidx = cv.take<unsigned>(nw), *sidx2 = cv.take<unsigned>(nw);
+    unsigned *chunks = cv.take<unsigned>((nw + 1023) / 1024), *counts = cv.take<unsigned>(3);
+    unsigned *hist = cv.take<unsigned>(32);
+    unsigned long long *total = cv.take<unsigned long long>(1);
+    if (!cv.ok)
+        return fail(MKDE_E_RANGE, "%lld points need %zu bytes of scratch, more than the %u available", (long long)n,
+                    cv.used, MKDE_SCRATCH_BYTES);
+    CHK(prep(c));
+    PointIndex I = index_of(c);
+    DensityWork W{cost, parts, item0, slot0, sidx, sidx2, chunks, total, counts, hist};
+    HIPCHK(mkde_dev::point_plan(c->stream, I, 0, c->h_fixed, nullptr, 0, W, c->tune));
+    unsigned cnt[3];   // items, split slots, split waves
+    CHK(copy_d2h(c, cnt, counts, sizeof cnt));
+    int4 *items = cv.take<int4>(cnt[0]);
+    int *order = cv.take<int>(cnt[0]);
+    int4 *splits = cv.take<int4>(cnt[2]);
+    double *partials = cv.take<double>((size_t)cnt[1] * 64);
+    if (!cv.ok)
+        return fail(MKDE_E_RANGE,
+                    "the work list of %lld points (%u items, %u split parts) needs %zu bytes of scratch, more than "
+                    "the %u available",
+                    (long long)n, cnt[0], cnt[1], cv.used, MKDE_SCRATCH_BYTES);
+    HIPCHK(mkde_dev::density_items(c->stream, (int)nw, W, (int)cnt[0], items, splits, order, nullptr));
+    HIPCHK(mkde_dev::point_density(c->stream, I, loo ? 1 : 0, den_s, c->h_fixed, order, items, cost, partials,
+                                   (int)cnt[0]));
+    HIPCHK(mkde_dev::point_merge(c->stream, n, 1, 1, (int)cnt[2], splits, partials, den_s, nullptr));
+    HIPCHK(mkde_dev::unsort_f32(c->stream, n, c->perm.p, den_s, den_dev));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return MKDE_OK;
 }
 

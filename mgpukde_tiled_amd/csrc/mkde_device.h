@@ -11,6 +11,10 @@
 #define MKDE_EPS_DEN 0.00000000000001f
 // candidate bandwidths per batched leave-one-out pass
 #define MKDE_LM_K 16
+// static device scratch of the allocation-free entry point (bytes per device)
+// and the device ordinals it is resolved for
+#define MKDE_SCRATCH_BYTES (32u << 20)
+#define MKDE_MAX_DEVICES 64
 
 // Launch-shape settings of one context: defaults from the environment
 // (default_tuning), changed by mkde_set_option or the autotuner.  They set how

@@ -32,6 +32,7 @@
 #include <float.h>
 #include <limits.h>
 #include <algorithm>
+#include <mutex>
 #include <stdint.h>
 #include <stdlib.h>
 
@@ -2335,6 +2336,39 @@ __global__ void k_fill_f32(int64_t n, float v, float *dst)
 
 extern "C++" {
 namespace mkde_dev {
+
+// ------------------------------------------------------------- scratch --
+// Device memory of the allocation-free entry point
+// (mkde_density_at_points_noalloc): the point kernels' work list, their FP64
+// partial sums and the bin-order densities are carved from this file-scope
+// array instead of buffers the library allocates.  Every device the library
+// runs on holds one copy of it until the process ends; all calls on a device
+// share it.  Host code never passes the array's name to a kernel (on the host
+// it is a stand-in, not a device address): scratch_address() resolves the
+// device address once per device, with that device current, and callers use
+// that pointer.
+static __device__ __align__(256) unsigned char scratch[MKDE_SCRATCH_BYTES];
+
+hipError_t scratch_address(int device, void **out)
+{
+    static std::mutex mu;
+    static void *addr[MKDE_MAX_DEVICES] = {};
+    if (!out || device < 0 || device >= MKDE_MAX_DEVICES) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;   // the caller makes the device current
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[device] = p;
+    }
+    *out = addr[device];
+    return hipSuccess;
+}
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 static inline unsigned nblk_(int64_t n) { return nblk(n, 256); }
