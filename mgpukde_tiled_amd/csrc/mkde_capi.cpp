@@ -58,12 +58,13 @@ hipError_t dist_tree(hipStream_t, int64_t, const float *, const float *, const i
 hipError_t ec_active(hipStream_t, int64_t, const GridDesc &, const float *, const float *, const double *,
                      const float *, float, const float *, float *, int *, int *, int, int);
 int ec_wide_max();
+int ec_box_cols(const GridDesc &, float);
 Tuning default_tuning();
 hipError_t centres(hipStream_t, const GridDesc &, float *, float *);
 hipError_t ec(hipStream_t, int64_t, int, int, int, const GridDesc &, const MaskRuns &, const float *, const float *,
               const float *, const float *, const float *, float, const int *, float *, unsigned long long *, int);
 hipError_t ec_dev(hipStream_t, int64_t, const GridDesc &, const MaskRuns &, const float *, const float *, const float *,
-                  const float *, const float *, const int *, const int *, float *, int);
+                  const float *, const float *, const int *, const int *, float *, int, int, int);
 hipError_t count_pairs(hipStream_t, const GridDesc &, const MaskRuns &, const PointIndex &, unsigned long long *);
 int64_t slot_words();
 hipError_t fold_slots(hipStream_t, unsigned long long *);
@@ -848,6 +849,16 @@ static int need_h(mkde_ctx *c)
 
 // ------------------------------------------------------- edge correction --
 
+// Widest box the wide EC kernel takes: ec_wide_max() columns of LDS, or for
+// a fixed bandwidth hf > 0 the bound of its boxes, so that a host-sized and a
+// graph-captured pass classify every point alike and the graph's wide launch
+// reserves no more LDS than the bandwidth can use.
+static int ec_wide_cols(const mkde_ctx *c, float hf)
+{
+    const int wmax = mkde_dev::ec_wide_max();
+    return hf > 0.0f ? std::min(wmax, mkde_dev::ec_box_cols(c->mg, hf)) : wmax;
+}
+
 // EC weights for bandwidth array h (original order) or the fixed value hf > 0,
 // written to ew_out (original order).
 static int run_ec(mkde_ctx *c, const float *h, float hf, float *ew_out, unsigned long long *n_eval,
@@ -862,7 +873,7 @@ static int run_ec(mkde_ctx *c, const float *h, float hf, float *ew_out, unsigned
     HIPCHK(hipMemsetAsync(c->ucount.p, 0, nw * sizeof(unsigned long long), c->stream));
     HIPCHK(mkde_dev::ec_active(c->stream, c->n, c->mg, c->x.p, c->y.p, c->dist.p, h, hf, nullptr, ew_out, c->list.p,
                                c->icount.p, std::min(c->tune.ec_wide_min, mkde_dev::ec_wide_max()),
-                               mkde_dev::ec_wide_max()));
+                               ec_wide_cols(c, hf)));
     int cnt3[3] = {0, 0, 0};   // narrow points, wide points, widest box
     CHK(copy_d2h(c, cnt3, c->icount.p, sizeof cnt3));
     MaskRuns R{c->run_ptr.p, c->run_se.p};
@@ -1142,7 +1153,8 @@ struct BandwidthGuard {
 // path, so the log-likelihoods are the same bit for bit.
 static const int64_t MKDE_SEARCH_GRAPH_MAX_PARTIALS = (int64_t)1 << 30;   // bytes of split-part sums
 
-static int search_enqueue(mkde_ctx *c, int K, int ec_on, const PointIndex &I, int64_t items_max, int64_t nw)
+static int search_enqueue(mkde_ctx *c, int K, int ec_on, const PointIndex &I, int64_t items_max, int64_t nw,
+                          const std::vector<int> &wcols)
 {
     const int64_t n = c->n;
     MaskRuns R{c->run_ptr.p, c->run_se.p};
@@ -1155,13 +1167,13 @@ static int search_enqueue(mkde_ctx *c, int K, int ec_on, const PointIndex &I, in
         for (int j = 0; j < kg; ++j) {
             const float *hd = c->s_h.p + g0 + j;
             if (ec_on) {
+                const int cw_narrow = std::min(c->tune.ec_wide_min, mkde_dev::ec_wide_max());
                 HIPCHK(mkde_dev::fill_f32(c->stream, 3, 0.0f, (float *)c->icount.p));   // zero counts (int bits)
                 HIPCHK(mkde_dev::ec_active(c->stream, n, c->mg, c->x.p, c->y.p, c->dist.p, nullptr, 0.0f, hd,
-                                           c->ewk.p, c->list.p, c->icount.p,
-                                           std::min(c->tune.ec_wide_min, mkde_dev::ec_wide_max()),
-                                           mkde_dev::ec_wide_max()));
+                                           c->ewk.p, c->list.p, c->icount.p, cw_narrow, wcols[g0 + j]));
                 HIPCHK(mkde_dev::ec_dev(c->stream, n, c->mg, R, c->colx.p, c->rowy.p, c->x.p, c->y.p, hd,
-                                        c->icount.p, c->list.p, c->ewk.p, c->tune.ec_chunk));
+                                        c->icount.p, c->list.p, c->ewk.p, c->tune.ec_chunk, cw_narrow,
+                                        wcols[g0 + j]));
             }
             HIPCHK(mkde_dev::coef_k_dev(c->stream, n, c->perm.p, c->w.p, c->ewk.p, hd,
                                         c->coefk.p + (MKDE_LM_K - kg) + j, MKDE_LM_K));
@@ -1223,6 +1235,10 @@ static int loo_loglik_graph(mkde_ctx *c, int K, const std::vector<int> &ord, con
     std::vector<float> hs(K);
     std::vector<double> a((size_t)G * MKDE_LM_K, 0.0), t((size_t)G * MKDE_LM_K, -1.0);
     for (int k = 0; k < K; ++k) hs[k] = h_cand[ord[k]];
+    // each candidate's widest wide EC box: the LDS of its wide launch (part
+    // of the key, so new candidates that need other sizes recapture)
+    std::vector<int> wcols(K);
+    for (int k = 0; k < K; ++k) wcols[k] = ec_wide_cols(c, hs[k]);
     for (int g0 = 0, gi = 0; g0 < K; g0 += MKDE_LM_K, ++gi) {
         const int kg = std::min(MKDE_LM_K, K - g0);
         for (int j = 0; j < kg; ++j) {
@@ -1263,6 +1279,7 @@ static int loo_loglik_graph(mkde_ctx *c, int K, const std::vector<int> &ord, con
                               (uint64_t)(uintptr_t)c->pw_partials.p, (uint64_t)(uintptr_t)c->partial.p,
                               (uint64_t)c->tune.ec_chunk, (uint64_t)c->tune.ec_wide_min, c->tune.loo_target};
     std::vector<uint64_t> key(std::begin(key_v), std::end(key_v));
+    if (ec_on) key.insert(key.end(), wcols.begin(), wcols.end());
     if (!c->s_exec || c->s_key != key) {
         if (c->s_exec) {
             (void)hipGraphExecDestroy(c->s_exec);
@@ -1271,7 +1288,7 @@ static int loo_loglik_graph(mkde_ctx *c, int K, const std::vector<int> &ord, con
         c->s_key.clear();
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        const int st = search_enqueue(c, K, ec_on, I, items_max, nw);
+        const int st = search_enqueue(c, K, ec_on, I, items_max, nw, wcols);
         const hipError_t e = hipStreamEndCapture(c->stream, &graph);
         if (st != MKDE_OK) {
             if (graph) (void)hipGraphDestroy(graph);

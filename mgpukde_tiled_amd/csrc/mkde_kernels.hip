@@ -1864,7 +1864,8 @@ struct ColTable {
 };
 struct ColCompute {
     double xll, cs;
-    __device__ __forceinline__ float operator()(int64_t c) const { return cell_x(xll, cs, c); }
+    // columns of one raster row fit in 32 bits (one-instruction conversion)
+    __device__ __forceinline__ float operator()(int64_t c) const { return cell_x(xll, cs, (int64_t)(int)c); }
 };
 
 template <class COL>
@@ -1878,10 +1879,11 @@ __device__ __forceinline__ bool pred_col(const COL &colx, int64_t c, float px, d
 // are inside the cut-off; lo > hi when empty.  cn is the column of [cl, cu]
 // nearest to px (|dx| is smallest there, and the predicate is monotone in the
 // distance from cn on either side).  The first guess (FP32 square root,
-// multiplication by 1/cs) only seeds the search: the five predicates that
-// confirm it (cn, both ends, one step beyond each end) are evaluated together,
-// so their centre loads are in flight at once; when the seed is off, the ends
-// are moved by the exact predicate until they are the last columns inside.
+// multiplication by 1/cs) only seeds the search: the four predicates that
+// confirm it (both ends, one step beyond each end) are evaluated together, so
+// table loads of their centres are in flight at once; when the seed is off,
+// the row is empty unless cn is inside, and the ends are moved by the exact
+// predicate until they are the last columns inside.
 template <class COL>
 __device__ __forceinline__ void row_interval(const COL &colx, double xll, double inv_cs, int cl, int cu, int cn,
                                              float px, double dy2, double thr, int &lo, int &hi)
@@ -1893,17 +1895,17 @@ __device__ __forceinline__ void row_interval(const COL &colx, double xll, double
     int h_e = (int)fmin(fmax(fh, (double)cn), (double)cu);
     int l_e = (int)fmin(fmax(fl, (double)cl), (double)cn);
     const int h1 = h_e < cu ? h_e + 1 : h_e, l1 = l_e > cl ? l_e - 1 : l_e;
-    const bool p_n = pred_col(colx, cn, px, dy2, thr);
     const bool p_h = pred_col(colx, h_e, px, dy2, thr);
     const bool p_h1 = pred_col(colx, h1, px, dy2, thr) & (h_e < cu);
     const bool p_l = pred_col(colx, l_e, px, dy2, thr);
     const bool p_l1 = pred_col(colx, l1, px, dy2, thr) & (l_e > cl);
-    if (!p_n) {
-        lo = 1;
-        hi = 0;
-        return;
-    }
     if (!(p_h & !p_h1 & p_l & !p_l1)) {
+        // cn is inside whenever any column is (p_h implies it)
+        if (!pred_col(colx, cn, px, dy2, thr)) {
+            lo = 1;
+            hi = 0;
+            return;
+        }
         while (h_e < cu && pred_col(colx, h_e + 1, px, dy2, thr)) ++h_e;
         while (h_e > cn && !pred_col(colx, h_e, px, dy2, thr)) --h_e;
         while (l_e > cl && pred_col(colx, l_e - 1, px, dy2, thr)) --l_e;
@@ -1980,31 +1982,73 @@ __device__ __forceinline__ int64_t first_run_ending_at_or_after(const int2 *runs
 // interval of the row (row_interval) clipped to the columns [ca, cb] whose
 // prefix sums of Ex are in s_pd, intersected with the row's valid runs;
 // acc += exp2(dy2 a) * (sum of Ex over those cells), cnt += their number.
+// Row and column centres are computed (cell_y / cell_x, the values of the
+// mask's centre tables) rather than loaded, and the row's run range [rs, re)
+// and, for a single-run row, its run r0 come loaded ahead by ec_rows, so the
+// row's arithmetic waits on no memory.
 __device__ __forceinline__ void ec_row(const EcArgs &A, const double *s_pd, int ca, int cb, int cl, int cu, int cn,
-                                       int r, float px, float py, double thr, double a, double xll, double inv_cs,
+                                       int r, int64_t rs, int64_t re, int2 r0, float px, float py, double thr,
+                                       double a, double xll, double inv_cs, double yll, double cs, double nrm05,
                                        double &acc, unsigned long long &cnt)
 {
-    const float cy = A.rowy[r];
+    const float cy = cell_y(yll, cs, nrm05, r);
     const double dy2 = sqd(cy - py);
     if (!(dy2 < thr)) return;
-    // the row's run range is loaded alongside the interval search
-    const int64_t rs = A.runs.run_ptr[r], re = A.runs.run_ptr[r + 1];
     int lo, hi;
-    row_interval(ColTable{A.colx}, xll, inv_cs, cl, cu, cn, px, dy2, thr, lo, hi);
+    row_interval(ColCompute{xll, cs}, xll, inv_cs, cl, cu, cn, px, dy2, thr, lo, hi);
     if (lo < ca) lo = ca;
     if (hi > cb) hi = cb;
     if (lo > hi) return;
     double seg = 0.0;
-    for (int64_t k = first_run_ending_at_or_after(A.runs.run_se, rs, re, lo); k < re; ++k) {
-        const int2 rr = A.runs.run_se[k];
-        if (rr.x > hi) break;
-        const int s = rr.x > lo ? rr.x : lo;
-        const int e = rr.y < hi ? rr.y : hi;
-        seg += s_pd[e - ca] - (s > ca ? s_pd[s - 1 - ca] : 0.0);
-        cnt += (unsigned long long)(e - s + 1);
+    if (re - rs == 1) {
+        const int s = r0.x > lo ? r0.x : lo;
+        const int e = r0.y < hi ? r0.y : hi;
+        if (s <= e) {
+            seg = s_pd[e - ca] - (s > ca ? s_pd[s - 1 - ca] : 0.0);
+            cnt += (unsigned long long)(e - s + 1);
+        }
+    } else {
+        for (int64_t k = first_run_ending_at_or_after(A.runs.run_se, rs, re, lo); k < re; ++k) {
+            const int2 rr = A.runs.run_se[k];
+            if (rr.x > hi) break;
+            const int s = rr.x > lo ? rr.x : lo;
+            const int e = rr.y < hi ? rr.y : hi;
+            seg += s_pd[e - ca] - (s > ca ? s_pd[s - 1 - ca] : 0.0);
+            cnt += (unsigned long long)(e - s + 1);
+        }
     }
     const float ey = fast_exp2((float)(dy2 * a));
     acc = __builtin_fma(seg, (double)ey, acc);
+}
+
+// Rows r, r + step, ... <= ru of a point's integral (ec_row each).  The run
+// range of the next row and the single run of the current one are loaded at
+// the top of an iteration, so their latency hides behind the row's interval
+// search.
+__device__ __forceinline__ void ec_rows(const EcArgs &A, const double *s_pd, int ca, int cb, int cl, int cu, int cn,
+                                        int r, int ru, int step, float px, float py, double thr, double a,
+                                        double &acc, unsigned long long &cnt)
+{
+    const GridDesc &g = A.g;
+    const double xll = g.xll, yll = g.yll, cs = g.cs, inv_cs = 1.0 / cs, nrm05 = (double)g.nrows - 0.5;
+    int64_t rs = 0, re = 0;
+    if (r <= ru) {
+        rs = A.runs.run_ptr[r];
+        re = A.runs.run_ptr[r + 1];
+    }
+    for (; r <= ru; r += step) {
+        int2 r0 = make_int2(1, 0);
+        if (re - rs == 1) r0 = A.runs.run_se[rs];
+        const int rn = r + step;
+        int64_t rs_n = 0, re_n = 0;
+        if (rn <= ru) {
+            rs_n = A.runs.run_ptr[rn];
+            re_n = A.runs.run_ptr[rn + 1];
+        }
+        ec_row(A, s_pd, ca, cb, cl, cu, cn, r, rs, re, r0, px, py, thr, a, xll, inv_cs, yll, cs, nrm05, acc, cnt);
+        rs = rs_n;
+        re = re_n;
+    }
 }
 
 // EC weight of point p by one wave: column chunks of EC_CW with the FP64
@@ -2014,7 +2058,7 @@ __device__ __forceinline__ void ec_point(const EcArgs &A, double *s_pd, int p)
 {
     const int lane = threadIdx.x;
     const GridDesc g = A.g;
-    const double xll = g.xll, cs = g.cs, inv_cs = 1.0 / cs;
+    const double xll = g.xll, cs = g.cs;
     const float px = A.x[p], py = A.y[p];
     const float hh = ec_bandwidth(A, p);
     const double h2 = (double)(float)(hh * hh);
@@ -2058,8 +2102,7 @@ __device__ __forceinline__ void ec_point(const EcArgs &A, double *s_pd, int p)
             }
             __syncthreads();
             // rows
-            for (int r = rl + lane; r <= ru; r += WAVE)
-                ec_row(A, s_pd, ca, cb, cl, cu, cn, r, px, py, thr, a, xll, inv_cs, acc, cnt);
+            ec_rows(A, s_pd, ca, cb, cl, cu, cn, rl + lane, ru, WAVE, px, py, thr, a, acc, cnt);
             __syncthreads();
         }
     }
@@ -2097,7 +2140,7 @@ __device__ __forceinline__ void ec_point_wide(const EcArgs &A, double *s_pd, dou
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const GridDesc g = A.g;
-    const double xll = g.xll, cs = g.cs, inv_cs = 1.0 / cs;
+    const double xll = g.xll, cs = g.cs;
     const float px = A.x[p], py = A.y[p];
     const float hh = ec_bandwidth(A, p);
     const double h2 = (double)(float)(hh * hh);
@@ -2138,7 +2181,7 @@ __device__ __forceinline__ void ec_point_wide(const EcArgs &A, double *s_pd, dou
     __syncthreads();
     double acc = 0.0;
     unsigned long long cnt = 0;
-    for (int r = rl + tid; r <= ru; r += 256) ec_row(A, s_pd, cl, cu, cl, cu, cn, r, px, py, thr, a, xll, inv_cs, acc, cnt);
+    ec_rows(A, s_pd, cl, cu, cl, cu, cn, rl + tid, ru, 256, px, py, thr, a, acc, cnt);
     for (int d = 32; d >= 1; d >>= 1) {
         acc += __shfl_xor(acc, d, WAVE);
         cnt += __shfl_xor(cnt, d, WAVE);
@@ -2613,6 +2656,18 @@ hipError_t centres(hipStream_t s, const GridDesc &g, float *colx, float *rowy)
 // Widest box k_ec_wide takes: its prefix sums fill at most 64,000 B of LDS.
 int ec_wide_max() { return 8000; }
 
+// Most columns of the NARROW box of a point at the fixed bandwidth h on grid
+// g, for every point within 3h of a valid cell (narrow_box: the box spans
+// 6h / cs columns between two truncations, so at most floor(6h / cs) + 2;
+// one more for the rounding of the double expressions).  A point whose box is
+// wider (only one lying more than 3h left of the raster, whose box the size_t
+// arithmetic stretches to the whole row) goes to the chunked k_ec.
+int ec_box_cols(const GridDesc &g, float h)
+{
+    const double w = std::floor(6.0 * (double)h / g.cs + 1e-6) + 3.0;
+    return (int)std::min<double>((double)g.ncols, std::max(w, 1.0));
+}
+
 static long long env_ll(const char *name, long long dflt)
 {
     const char *e = getenv(name);
@@ -2668,20 +2723,27 @@ hipError_t ec(hipStream_t s, int64_t n, int n_narrow, int n_wide, int max_w, con
 // The same EC pass with every size on the device (no host read-back, so the
 // launches can be captured in a graph): the bandwidth is *h_dev, the list
 // lengths count[0] (narrow) and count[1] (wide) as k_ec_active left them.
-// Fixed grids loop over the list; the wide kernel always takes the largest
-// box's LDS (ec_wide_max columns).
+// Fixed grids loop over the list.  w_wide is the cw_wide k_ec_active was given
+// (the widest wide box): the wide kernel takes that much LDS, as many
+// workgroups as fit on 256 CUs, and is left out when no box can be wide
+// (w_wide <= cw_narrow).
 hipError_t ec_dev(hipStream_t s, int64_t n, const GridDesc &g, const MaskRuns &runs, const float *colx,
                   const float *rowy, const float *x, const float *y, const float *h_dev, const int *count,
-                  const int *list, float *ew, int cw)
+                  const int *list, float *ew, int cw, int cw_narrow, int w_wide)
 {
     if (n <= 0) return hipSuccess;
+    if (w_wide > ec_wide_max()) return hipErrorInvalidValue;
     EcArgs A{g, runs, colx, rowy, x, y, nullptr, 0.0f, h_dev, count, list, ew, nullptr};
     const unsigned gn = (unsigned)std::min<int64_t>(n, 256 * 32);   // 8 waves per SIMD
-    const unsigned gw = (unsigned)std::min<int64_t>(n, 256 * 2);    // 2 workgroups per CU (64,000 B of LDS)
     if (cw == 512) hipLaunchKernelGGL(k_ec<512>, dim3(gn), dim3(64), 0, s, A);
     else if (cw == 1024) hipLaunchKernelGGL(k_ec<1024>, dim3(gn), dim3(64), 0, s, A);
     else hipLaunchKernelGGL(k_ec<2048>, dim3(gn), dim3(64), 0, s, A);
-    hipLaunchKernelGGL(k_ec_wide, dim3(gw), dim3(256), (size_t)ec_wide_max() * sizeof(double), s, A, n);
+    if (w_wide > cw_narrow) {
+        const size_t lds = (size_t)w_wide * sizeof(double);
+        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160u << 10) / (lds + 256)));
+        const unsigned gw = (unsigned)std::min<int64_t>(n, 256 * per_cu);
+        hipLaunchKernelGGL(k_ec_wide, dim3(gw), dim3(256), lds, s, A, n);
+    }
     return hipGetLastError();
 }
 
