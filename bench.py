@@ -580,6 +580,8 @@ def serve_jobs(count, seed=20260005, max_points=5_000_000, max_side=8192):
                                    hole_max=max(3, side // 10), seed=seed + 7 * i)
         jobs.append(dict(x=x, y=y, g=M.grid(side, side, 0.0, 0.0, 1.0), mask=mask, hoption=hopt, n=len(x),
                          side=side))
+        if (i + 1) % 200 == 0:
+            log("serve:   %d of %d jobs generated" % (i + 1, count))
     return jobs
 
 
@@ -609,6 +611,8 @@ def run_served(jobs, slots, arrivals):
                     submitted.wait(0.001)
                 res, out = srv.wait(tickets[i])
                 results[i] = (res, _digest(out))
+                if (i + 1) % 100 == 0:
+                    log("serve:   %d of %d jobs done" % (i + 1, len(jobs)))
 
         th = threading.Thread(target=collect)
         th.start()
@@ -632,10 +636,12 @@ def run_sequential(jobs, plain=False):
     out = []
     with M.Server(slots=1) as srv:
         t0 = time.perf_counter()
-        for j in jobs:
+        for i, j in enumerate(jobs):
             res, o = srv.wait(_submit(srv, j, plain))
             assert res["status"] == 0, res["message"]
             out.append((res, _digest(o)))
+            if (i + 1) % 100 == 0:
+                log("serve:   %d of %d jobs done one at a time" % (i + 1, len(jobs)))
         t = time.perf_counter() - t0
     return out, t
 
@@ -842,6 +848,9 @@ def main():
                     help="jobs of the serving sample in the benchmark line (0: none)")
     ap.add_argument("--slots", type=int, default=4, help="serve: worker slots")
     ap.add_argument("--qps", type=float, default=0.0, help="serve: one arrival rate instead of fractions of capacity")
+    ap.add_argument("--no-plain", action="store_true", help="serve: skip the one-candidate-per-pass baseline")
+    ap.add_argument("--fractions", default="0.5,0.8,0.95",
+                    help="serve: arrival rates as fractions of the measured capacity (empty: capacity only)")
     ap.add_argument("--workload", default="adaptive", choices=["adaptive", "search", "tiles", "serve", "copy", "rates"],
                     help="adaptive: the benchmark line; the others measure one workload on its own")
     args = ap.parse_args()
@@ -862,7 +871,9 @@ def main():
         print(json.dumps(leg_tiles(args, with_oracle=not args.no_accuracy, stats=args.stats)), flush=True)
         return
     if args.workload == "serve":
-        print(json.dumps(bench_serving(args.jobs, args.slots, qps=args.qps)), flush=True)
+        fr = tuple(float(v) for v in args.fractions.split(",") if v.strip())
+        print(json.dumps(bench_serving(args.jobs, args.slots, fractions=fr, qps=args.qps, plain=not args.no_plain)),
+              flush=True)
         return
     out, O_cpu = bench_adaptive(args)
     if not args.no_legs:

@@ -239,7 +239,7 @@ int mkde_density_stats(mkde_ctx *ctx, int enable, int64_t *counters);
  *                    the same; 0: the host-sized launches (same results)
  *   "density_order"  1 (default): density blocks visited heaviest first with
  *                    heavy blocks split; 0: raster order, no split
- *   "points_per_bin" target points per bin of the point index (default 32),
+ *   "points_per_bin" target points per bin of the point index (default 64),
  *                    used by the next mkde_points_upload
  *   "ec_chunk"       512, 1024 (default) or 2048 columns per EC prefix chunk
  *   "ec_wide_min"    boxes wider than this (default: ec_chunk) use the wide

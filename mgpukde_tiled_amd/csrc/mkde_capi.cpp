@@ -249,7 +249,7 @@ struct mkde_ctx {
     // work list of the density kernel: heavy blocks split into parts,
     // items visited heaviest first
     bool dt_order_on = true;
-    int pts_per_bin = 32;   // target points per bin of the point index
+    int pts_per_bin = 64;   // target points per bin of the point index
     DevBuf<unsigned> blk_cost, blk_parts, blk_item0, blk_slot0, blk_sidx, blk_sidx2, blk_chunks, blk_counts,
         blk_hist;
     DevBuf<unsigned long long> blk_total;
@@ -391,7 +391,7 @@ static int copy_d2h(mkde_ctx *c, void *dst, const void *src, size_t bytes)
 
 // ---------------------------------------------------------- point index --
 //
-// Uniform bins over the points' bounding box, about 32 points per bin, at most
+// Uniform bins over the points' bounding box, about 64 points per bin, at most
 // 4M bins; a stable counting sort gives the bin order (ties keep the caller's
 // order, so every downstream summation order is deterministic).
 
@@ -1914,7 +1914,7 @@ extern "C" int mkde_autotune_job(mkde_ctx *c, const mkde_job_opts *o, int reps, 
         opts.push_back({"ec_chunk", {512, 1024, 2048}});
         opts.push_back({"ec_wide_min", {512, 1024, 2048, 4096}});
     }
-    if (o->hoption != -1) opts.push_back({"points_per_bin", {16, 32, 64}});
+    if (o->hoption != -1) opts.push_back({"points_per_bin", {32, 64, 128}});
     std::string rep = "{\"workload\": \"job\", ";
     double ms = 0.0;
     CHK(autotune_run(c, opts, reps, [&]() { return mkde_run_job(c, o, nullptr); }, rep, &ms));
@@ -1939,7 +1939,7 @@ extern "C" int mkde_autotune_search(mkde_ctx *c, int K, const float *h_cand, int
         opts.push_back({"ec_chunk", {512, 1024, 2048}});
         opts.push_back({"ec_wide_min", {512, 1024, 2048, 4096}});
     }
-    opts.push_back({"points_per_bin", {16, 32, 64}});
+    opts.push_back({"points_per_bin", {32, 64, 128}});
     std::string rep = "{\"workload\": \"search\", ";
     double ms = 0.0;
     CHK(autotune_run(c, opts, reps, [&]() { return mkde_loo_loglik(c, K, h_cand, ec_on, L.data()); }, rep, &ms));
