@@ -1300,33 +1300,53 @@ struct LooMultiArgs {
     const unsigned *n_items_dev;   // non-null: the item count on the device (n_items: grid bound)
 };
 
-// Slot K of a source's candidate walk (then K-1, ... while a lane is inside):
-// acc[K] += (in ? exp2(d2f a_K) : 0) * coef_K, with slot K-1's mask made
-// before slot K's term.  While every lane is inside (the largest candidates
-// of a source that is near the whole wave) the term needs no select; the
-// select of a lane inside adds the same value, so both forms give the same
-// sums.  One wait state after the transcendental before its result is read;
-// a mask is at least two instructions old when v_cndmask reads it.
+// Slots K .. K-3 of a source's candidate walk (then the next four while a
+// lane is inside): `in` is slot K's lane mask; the block forms the masks of
+// slots K-1 .. K-3 and of K-4 (the next block's first) at its start, adds
+// the four terms, without selects when every lane is inside slot K-3 (then
+// inside all four), else each term selected by its mask.  A slot no lane is
+// inside adds exp2(..) * coef selected to +0 (acc + 0 = acc), so finishing a
+// block past the last slot with a lane inside changes no sum, and the walk
+// ends after the first block whose smallest slot is empty (the cut-off is
+// monotone in h).  Two scalar tests per block instead of two per slot.  The
+// four exponentials issue before their first use, so no wait state is
+// needed after a transcendental; a mask is at least two instructions old
+// when v_cndmask reads it.
+#define MKDE_LT(I) "v_mul_f32_e32 %[t" #I "], %[d2f], %[a" #I "]\n\t"
+#define MKDE_LE(I) "v_exp_f32_e32 %[t" #I "], %[t" #I "]\n\t"
+#define MKDE_LS(I) "v_cndmask_b32_e64 %[t" #I "], 0, %[t" #I "], %[m" #I "]\n\t"
+#define MKDE_LF(I) "v_fmac_f32_e32 %[c" #I "], %[f" #I "], %[t" #I "]\n\t"
 template <int K>
-__device__ __forceinline__ void loo_step(uint64_t in, double d2, float d2f, const double (&thr)[LM_K],
-                                         const float (&af)[LM_K], const float (&cf)[LM_K], float (&acc)[LM_K])
+__device__ __forceinline__ void loo_block(uint64_t in, double d2, float d2f, const double (&thr)[LM_K],
+                                          const float (&af)[LM_K], const float (&cf)[LM_K], float (&acc)[LM_K])
 {
+    static_assert(K % 4 == 3, "blocks of four slots");
     if (in == 0) return;
-    uint64_t next = 0;
-    if constexpr (K > 0) asm("v_cmp_gt_f64_e64 %0, %1, %2" : "=s"(next) : "s"(thr[K - 1]), "v"(d2));
-    float t;
-    asm("v_mul_f32_e32 %0, %2, %3\n\t"
-        "v_exp_f32_e32 %0, %0\n\t"
-        "s_nop 0\n\t"
-        "s_cmp_eq_u64 %4, -1\n\t"
-        "s_cbranch_scc1 1f\n\t"
-        "v_cndmask_b32_e64 %0, 0, %0, %4\n"
-        "1:\n\t"
-        "v_fmac_f32_e32 %1, %5, %0"
-        : "=&v"(t), "+v"(acc[K])
-        : "v"(d2f), "v"(af[K]), "s"(in), "s"(cf[K])
-        : "scc");
-    if constexpr (K > 0) loo_step<K - 1>(next, d2, d2f, thr, af, cf, acc);
+    uint64_t m2, m1, m0, next = 0;
+    asm("v_cmp_gt_f64_e64 %0, %3, %6\n\t"
+        "v_cmp_gt_f64_e64 %1, %4, %6\n\t"
+        "v_cmp_gt_f64_e64 %2, %5, %6"
+        : "=&s"(m2), "=&s"(m1), "=&s"(m0)
+        : "s"(thr[K - 1]), "s"(thr[K - 2]), "s"(thr[K - 3]), "v"(d2));
+    if constexpr (K > 3) asm("v_cmp_gt_f64_e64 %0, %1, %2" : "=s"(next) : "s"(thr[K - 4]), "v"(d2));
+    float t0, t1, t2, t3;
+    if (m0 == ~(uint64_t)0) {
+        asm(MKDE_LT(0) MKDE_LT(1) MKDE_LT(2) MKDE_LT(3) MKDE_LE(0) MKDE_LE(1) MKDE_LE(2) MKDE_LE(3)
+                MKDE_LF(0) MKDE_LF(1) MKDE_LF(2) MKDE_LF(3)
+            : [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3), [c0] "+v"(acc[K]),
+              [c1] "+v"(acc[K - 1]), [c2] "+v"(acc[K - 2]), [c3] "+v"(acc[K - 3])
+            : [d2f] "v"(d2f), [a0] "v"(af[K]), [a1] "v"(af[K - 1]), [a2] "v"(af[K - 2]), [a3] "v"(af[K - 3]),
+              [f0] "s"(cf[K]), [f1] "s"(cf[K - 1]), [f2] "s"(cf[K - 2]), [f3] "s"(cf[K - 3]));
+    } else {
+        asm(MKDE_LT(0) MKDE_LT(1) MKDE_LT(2) MKDE_LT(3) MKDE_LE(0) MKDE_LE(1) MKDE_LE(2) MKDE_LE(3)
+                MKDE_LS(0) MKDE_LS(1) MKDE_LS(2) MKDE_LS(3) MKDE_LF(0) MKDE_LF(1) MKDE_LF(2) MKDE_LF(3)
+            : [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3), [c0] "+v"(acc[K]),
+              [c1] "+v"(acc[K - 1]), [c2] "+v"(acc[K - 2]), [c3] "+v"(acc[K - 3])
+            : [d2f] "v"(d2f), [a0] "v"(af[K]), [a1] "v"(af[K - 1]), [a2] "v"(af[K - 2]), [a3] "v"(af[K - 3]),
+              [f0] "s"(cf[K]), [f1] "s"(cf[K - 1]), [f2] "s"(cf[K - 2]), [f3] "s"(cf[K - 3]), [m0] "s"(in),
+              [m1] "s"(m2), [m2] "s"(m1), [m3] "s"(m0));
+    }
+    if constexpr (K > 3) loo_block<K - 4>(next, d2, d2f, thr, af, cf, acc);
 }
 
 // One source of the batched leave-one-out pass for the wave's 64 queries:
@@ -1364,7 +1384,7 @@ __device__ __forceinline__ void loo_source(const PtRec *pts, const float *coefk,
         : "s"(thr[LM_K - 1]), "v"(d2), "s"(cf[0]), "s"(cf[1]), "s"(cf[2]), "s"(cf[3]), "s"(cf[4]), "s"(cf[5]),
           "s"(cf[6]), "s"(cf[7]), "s"(cf[8]), "s"(cf[9]), "s"(cf[10]), "s"(cf[11]), "s"(cf[12]), "s"(cf[13]),
           "s"(cf[14]), "s"(cf[15]));
-    loo_step<LM_K - 1>(in, d2, d2f, thr, af, cf, acc);
+    loo_block<LM_K - 1>(in, d2, d2f, thr, af, cf, acc);
 }
 
 // Batched leave-one-out densities of up to 16 fixed bandwidths in one pass
@@ -2086,8 +2106,9 @@ __device__ __forceinline__ void ec_point(const EcArgs &A, double *s_pd, int p)
         for (int ca = cl; ca <= cu; ca += EC_CW) {
             const int cb = (ca + EC_CW - 1 < cu) ? ca + EC_CW - 1 : cu;
             const int w = cb - ca + 1;
-            // lane owns columns [ca + lane*per, ...)
-            const int per = (w + WAVE - 1) / WAVE;
+            // lane owns columns [ca + lane*per, ...); an odd per puts the
+            // lanes' FP64 prefix entries in distinct LDS banks
+            const int per = ((w + WAVE - 1) / WAVE) | 1;
             const int my0 = lane * per;
             // FP64 prefix (53-bit significand, more than the 48 of a
             // float-float pair; FP64 adds cost one VALU instruction each)
@@ -2095,7 +2116,7 @@ __device__ __forceinline__ void ec_point(const EcArgs &A, double *s_pd, int p)
             for (int t = 0; t < per; ++t) {
                 const int k = my0 + t;
                 if (k < w) {
-                    const float dx = A.colx[ca + k] - px;
+                    const float dx = ColCompute{xll, cs}(ca + k) - px;
                     const float ex = fast_exp2((float)(sqd(dx) * a));
                     run += (double)ex;
                     s_pd[k] = run;
@@ -2163,13 +2184,15 @@ __device__ __forceinline__ void ec_point_wide(const EcArgs &A, double *s_pd, dou
     narrow_box(g, px, py, hh, rl, ru, cl, cu);   // non-empty: k_ec_active checked
     const int cn = clampi((int64_t)floor(((double)px - xll) / cs), cl, cu);
     const int w = cu - cl + 1;
-    const int per = (w + 255) / 256;
+    // thread owns columns [cl + tid*per, ...); an odd per puts the threads'
+    // FP64 prefix entries in distinct LDS banks
+    const int per = ((w + 255) / 256) | 1;
     const int my0 = tid * per;
     double run = 0.0;
     for (int t = 0; t < per; ++t) {
         const int k = my0 + t;
         if (k < w) {
-            const float dx = A.colx[cl + k] - px;
+            const float dx = ColCompute{xll, cs}(cl + k) - px;
             const float ex = fast_exp2((float)(sqd(dx) * a));
             run += (double)ex;
             s_pd[k] = run;
