@@ -2005,25 +2005,17 @@ __device__ __forceinline__ int64_t first_run_ending_at_or_after(const int2 *runs
 // Row and column centres are computed (cell_y / cell_x, the values of the
 // mask's centre tables) rather than loaded, and the row's run range [rs, re)
 // and, for a single-run row, its run r0 come loaded ahead by ec_rows, so the
-// row's arithmetic waits on no memory.  dxl2 / dxr2 are the squared
-// distances of the box's end columns cl / cu (cover: both inside on the
-// point's own row); a row whose chord holds both ends is [cl, cu] whole (the
-// predicate is monotone from cn outward), with no interval search.
+// row's arithmetic waits on no memory.
 __device__ __forceinline__ void ec_row(const EcArgs &A, const double *s_pd, int ca, int cb, int cl, int cu, int cn,
                                        int r, int64_t rs, int64_t re, int2 r0, float px, float py, double thr,
                                        double a, double xll, double inv_cs, double yll, double cs, double nrm05,
-                                       bool cover, double dxl2, double dxr2, double &acc, unsigned long long &cnt)
+                                       double &acc, unsigned long long &cnt)
 {
     const float cy = cell_y(yll, cs, nrm05, r);
     const double dy2 = sqd(cy - py);
     if (!(dy2 < thr)) return;
     int lo, hi;
-    if (cover && __dadd_rn(dxl2, dy2) < thr && __dadd_rn(dxr2, dy2) < thr) {
-        lo = cl;
-        hi = cu;
-    } else {
-        row_interval(ColCompute{xll, cs}, xll, inv_cs, cl, cu, cn, px, dy2, thr, lo, hi);
-    }
+    row_interval(ColCompute{xll, cs}, xll, inv_cs, cl, cu, cn, px, dy2, thr, lo, hi);
     if (lo < ca) lo = ca;
     if (hi > cb) hi = cb;
     if (lo > hi) return;
@@ -2059,9 +2051,6 @@ __device__ __forceinline__ void ec_rows(const EcArgs &A, const double *s_pd, int
 {
     const GridDesc &g = A.g;
     const double xll = g.xll, yll = g.yll, cs = g.cs, inv_cs = 1.0 / cs, nrm05 = (double)g.nrows - 0.5;
-    const ColCompute colx{xll, cs};
-    const double dxl2 = sqd(colx(cl) - px), dxr2 = sqd(colx(cu) - px);
-    const bool cover = dxl2 < thr && dxr2 < thr;
     int64_t rs = 0, re = 0;
     if (r <= ru) {
         rs = A.runs.run_ptr[r];
@@ -2076,8 +2065,7 @@ __device__ __forceinline__ void ec_rows(const EcArgs &A, const double *s_pd, int
             rs_n = A.runs.run_ptr[rn];
             re_n = A.runs.run_ptr[rn + 1];
         }
-        ec_row(A, s_pd, ca, cb, cl, cu, cn, r, rs, re, r0, px, py, thr, a, xll, inv_cs, yll, cs, nrm05, cover, dxl2,
-               dxr2, acc, cnt);
+        ec_row(A, s_pd, ca, cb, cl, cu, cn, r, rs, re, r0, px, py, thr, a, xll, inv_cs, yll, cs, nrm05, acc, cnt);
         rs = rs_n;
         re = re_n;
     }
