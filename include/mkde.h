@@ -310,7 +310,12 @@ int mkde_device_memory(mkde_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes)
  * MKDE_H_SEARCH for searches --, edge correction, optimiser);
  * mkde_autotune_lookup applies a recorded choice to a context (MKDE_E_STATE
  * if the class has none).  report (may be NULL) receives a JSON text of every
- * timing and the choices; MKDE_E_RANGE if report_cap is too small. */
+ * timing and the choices; MKDE_E_RANGE if report_cap is too small.  Trying
+ * "points_per_bin" rebuilds the point index from the uploaded points (as
+ * mkde_points_upload: bandwidths, boundary distances and EC weights are
+ * reset); jobs with hoption -1 (caller's bandwidths) do not try it.  When a
+ * trial fails, its option is set back to its value before the trial and the
+ * error is returned. */
 int mkde_autotune_job(mkde_ctx *ctx, const mkde_job_opts *opts, int reps, char *report, int64_t report_cap);
 int mkde_autotune_search(mkde_ctx *ctx, int K, const float *h_cand, int ec_on, int reps, char *report,
                          int64_t report_cap);

@@ -1851,6 +1851,12 @@ int autotune_run(mkde_ctx *c, const std::vector<TuneOpt> &opts, int reps, const 
     snprintf(buf, sizeof buf, "\"baseline_ms\": %.4f, \"trials\": [", cur);
     rep += buf;
     bool first = true;
+    // an option takes a value; points_per_bin rebuilds the point index
+    auto apply = [&](const char *name, int64_t v) -> int {
+        CHK(mkde_set_option(c, name, v));
+        if (!strcmp(name, "points_per_bin")) CHK(mkde_points_upload(c, c->n, c->hx.data(), c->hy.data(), c->hw.data()));
+        return MKDE_OK;
+    };
     for (const TuneOpt &o : opts) {
         int64_t keep = 0;
         CHK(mkde_get_option(c, o.name, &keep));
@@ -1858,10 +1864,15 @@ int autotune_run(mkde_ctx *c, const std::vector<TuneOpt> &opts, int reps, const 
         double best_ms = cur;
         for (int64_t v : o.values) {
             if (v == keep) continue;
-            CHK(mkde_set_option(c, o.name, v));
-            if (!strcmp(o.name, "points_per_bin")) CHK(mkde_points_upload(c, c->n, c->hx.data(), c->hy.data(), c->hw.data()));
             double ms = 0.0;
-            CHK(timed(&ms));
+            int st = apply(o.name, v);
+            if (st == MKDE_OK) st = timed(&ms);
+            if (st != MKDE_OK) {
+                // the option keeps its value from before the failed trial
+                const std::string msg = g_err;
+                (void)apply(o.name, keep);
+                return fail(st, "%s", msg.c_str());
+            }
             snprintf(buf, sizeof buf, "%s{\"option\": \"%s\", \"value\": %lld, \"ms\": %.4f}", first ? "" : ", ", o.name,
                      (long long)v, ms);
             rep += buf;
@@ -1871,8 +1882,7 @@ int autotune_run(mkde_ctx *c, const std::vector<TuneOpt> &opts, int reps, const 
                 best_v = v;
             }
         }
-        CHK(mkde_set_option(c, o.name, best_v));
-        if (!strcmp(o.name, "points_per_bin")) CHK(mkde_points_upload(c, c->n, c->hx.data(), c->hy.data(), c->hw.data()));
+        CHK(apply(o.name, best_v));
         cur = best_ms;
     }
     rep += "], \"chosen\": {";
