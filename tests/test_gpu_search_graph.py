@@ -90,3 +90,20 @@ def test_graph_follows_new_points(ctx, scene):
     b_eager = ctx.LogLikelihood(hc, edge_correction=True)
     assert np.array_equal(b, b_eager)
     assert not np.array_equal(a, b)
+
+
+def test_graph_follows_launch_shape_options(ctx, scene):
+    # a changed split target or EC chunk recaptures the graph: each replay
+    # equals the host-sized launches with the same options
+    x, y, mask, g = scene
+    ctx.points_upload(x, y, None)
+    ctx.mask_upload(g, mask)
+    h0 = ctx.BandWidth_rule_of_thumb()
+    hc = candidates(h0, 24)
+    for opt, val in (("loo_split_target", 8192), ("ec_chunk", 512), ("ec_wide_min", 2048)):
+        keep = ctx.get_option(opt)
+        ctx.set_option(opt, val)
+        e, gr = eager_and_graph(ctx, hc)
+        assert np.array_equal(e, gr), opt
+        assert np.array_equal(ctx.LogLikelihood(hc, edge_correction=True), e), opt
+        ctx.set_option(opt, keep)
