@@ -1329,23 +1329,22 @@ __device__ __forceinline__ void loo_block(uint64_t in, double d2, float d2f, con
         : "=&s"(m2), "=&s"(m1), "=&s"(m0)
         : "s"(thr[K - 1]), "s"(thr[K - 2]), "s"(thr[K - 3]), "v"(d2));
     if constexpr (K > 3) asm("v_cmp_gt_f64_e64 %0, %1, %2" : "=s"(next) : "s"(thr[K - 4]), "v"(d2));
+    // one piece of assembly for both forms (a scalar branch inside it skips
+    // the selects when every lane is inside slot K-3), so the accumulators
+    // keep their registers whichever form runs
     float t0, t1, t2, t3;
-    if (m0 == ~(uint64_t)0) {
-        asm(MKDE_LT(0) MKDE_LT(1) MKDE_LT(2) MKDE_LT(3) MKDE_LE(0) MKDE_LE(1) MKDE_LE(2) MKDE_LE(3)
-                MKDE_LF(0) MKDE_LF(1) MKDE_LF(2) MKDE_LF(3)
-            : [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3), [c0] "+v"(acc[K]),
-              [c1] "+v"(acc[K - 1]), [c2] "+v"(acc[K - 2]), [c3] "+v"(acc[K - 3])
-            : [d2f] "v"(d2f), [a0] "v"(af[K]), [a1] "v"(af[K - 1]), [a2] "v"(af[K - 2]), [a3] "v"(af[K - 3]),
-              [f0] "s"(cf[K]), [f1] "s"(cf[K - 1]), [f2] "s"(cf[K - 2]), [f3] "s"(cf[K - 3]));
-    } else {
-        asm(MKDE_LT(0) MKDE_LT(1) MKDE_LT(2) MKDE_LT(3) MKDE_LE(0) MKDE_LE(1) MKDE_LE(2) MKDE_LE(3)
-                MKDE_LS(0) MKDE_LS(1) MKDE_LS(2) MKDE_LS(3) MKDE_LF(0) MKDE_LF(1) MKDE_LF(2) MKDE_LF(3)
-            : [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3), [c0] "+v"(acc[K]),
-              [c1] "+v"(acc[K - 1]), [c2] "+v"(acc[K - 2]), [c3] "+v"(acc[K - 3])
-            : [d2f] "v"(d2f), [a0] "v"(af[K]), [a1] "v"(af[K - 1]), [a2] "v"(af[K - 2]), [a3] "v"(af[K - 3]),
-              [f0] "s"(cf[K]), [f1] "s"(cf[K - 1]), [f2] "s"(cf[K - 2]), [f3] "s"(cf[K - 3]), [m0] "s"(in),
-              [m1] "s"(m2), [m2] "s"(m1), [m3] "s"(m0));
-    }
+    asm(MKDE_LT(0) MKDE_LT(1) MKDE_LT(2) MKDE_LT(3) MKDE_LE(0) MKDE_LE(1) MKDE_LE(2) MKDE_LE(3)
+        "s_cmp_eq_u64 %[m3], -1\n\t"
+        "s_cbranch_scc1 1f\n\t"
+        MKDE_LS(0) MKDE_LS(1) MKDE_LS(2) MKDE_LS(3)
+        "\n1:\n\t"
+        MKDE_LF(0) MKDE_LF(1) MKDE_LF(2) MKDE_LF(3)
+        : [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3), [c0] "+v"(acc[K]),
+          [c1] "+v"(acc[K - 1]), [c2] "+v"(acc[K - 2]), [c3] "+v"(acc[K - 3])
+        : [d2f] "v"(d2f), [a0] "v"(af[K]), [a1] "v"(af[K - 1]), [a2] "v"(af[K - 2]), [a3] "v"(af[K - 3]),
+          [f0] "s"(cf[K]), [f1] "s"(cf[K - 1]), [f2] "s"(cf[K - 2]), [f3] "s"(cf[K - 3]), [m0] "s"(in),
+          [m1] "s"(m2), [m2] "s"(m1), [m3] "s"(m0)
+        : "scc");
     if constexpr (K > 3) loo_block<K - 4>(next, d2, d2f, thr, af, cf, acc);
 }
 
