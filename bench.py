@@ -592,7 +592,13 @@ def _submit(srv, j, plain=False):
 
 
 def _digest(a):
-    return hashlib.blake2b(a.tobytes(), digest_size=16).hexdigest()
+    """128-bit hash of a result raster (xxh3: a few ms for a 64 MB raster;
+    blake2b, ~30x slower, when xxhash is missing)."""
+    try:
+        import xxhash
+        return xxhash.xxh3_128_hexdigest(memoryview(np.ascontiguousarray(a)).cast("B"))
+    except ImportError:
+        return hashlib.blake2b(memoryview(np.ascontiguousarray(a)).cast("B"), digest_size=16).hexdigest()
 
 
 def run_served(jobs, slots, arrivals):
@@ -634,15 +640,18 @@ def run_sequential(jobs, plain=False):
     import mgpukde_tiled_amd as M
 
     out = []
+    t_hash = 0.0   # hashing the results is the benchmark's check, not serving work
     with M.Server(slots=1) as srv:
         t0 = time.perf_counter()
         for i, j in enumerate(jobs):
             res, o = srv.wait(_submit(srv, j, plain))
             assert res["status"] == 0, res["message"]
+            th = time.perf_counter()
             out.append((res, _digest(o)))
+            t_hash += time.perf_counter() - th
             if (i + 1) % 100 == 0:
                 log("serve:   %d of %d jobs done one at a time" % (i + 1, len(jobs)))
-        t = time.perf_counter() - t0
+        t = time.perf_counter() - t0 - t_hash
     return out, t
 
 
