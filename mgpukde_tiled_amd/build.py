@@ -26,7 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + 
 # Host-only translation units use the HIP runtime API through its C++ headers.
 HOST_FLAGS = ["-D__HIP_PLATFORM_AMD__=1", "-I/opt/rocm/include"]
 
-HIP_SOURCES = ["mkde_kernels.hip"]
+HIP_SOURCES = ["mkde_kernels.hip", "mkde_bins.hip"]
 HOST_SOURCES = ["mkde_capi.cpp", "mkde_tiff.cpp", "mkde_serve.cpp"]
 
 

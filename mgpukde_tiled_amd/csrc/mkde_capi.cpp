@@ -16,6 +16,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -49,7 +50,13 @@ hipError_t loo_multi(hipStream_t, const PointIndex &, const float *, const doubl
 hipError_t point_merge(hipStream_t, int64_t, int, int, int, const int4 *, const double *, float *, const unsigned *);
 hipError_t range_count(hipStream_t, const PointIndex &, int64_t *);
 hipError_t range_fill(hipStream_t, const PointIndex &, const int64_t *, int32_t *);
+hipError_t bin_sort_bytes(int64_t, int, size_t *);
+hipError_t bin_points(hipStream_t, int64_t, const float *, const float *, const BinGrid &, int, unsigned *, unsigned *,
+                      int *, int *, void *, size_t, float *, float *, int *);
 hipError_t valid_cells(hipStream_t, int64_t, const float *, float, uint8_t *);
+hipError_t runs_count(hipStream_t, const GridDesc &, const uint8_t *, unsigned *, unsigned *, unsigned *, unsigned *,
+                      int64_t *);
+hipError_t runs_fill(hipStream_t, const GridDesc &, const uint8_t *, const unsigned *, int2 *);
 hipError_t boundary(hipStream_t, const GridDesc &, const uint8_t *, uint8_t *, float2 *, unsigned long long *);
 hipError_t dist_boundary(hipStream_t, int64_t, const float *, const float *, const float2 *, int64_t, double *);
 hipError_t bt_build(hipStream_t, const GridDesc &, const uint8_t *, uint4 *, float4 *, const BndTree &);
@@ -173,8 +180,11 @@ struct mkde_ctx {
     // cell-binned index
     BinGrid bg{};
     int nbins = 0;
-    std::vector<int> hperm;
+    std::vector<int> hperm;     // host copy of perm (host_perm), empty until needed
     DevBuf<int> perm;           // bin-order position -> original index
+    DevBuf<unsigned> bin_key, bin_skey;   // building the bin order
+    DevBuf<int> bin_idx;
+    DevBuf<uint8_t> bin_tmp;
     DevBuf<int> bin_start;      // nbins + 1
     DevBuf<float> xs, ys;       // bin order
     DevBuf<PtRec> pts;          // bin order, per-job constants
@@ -198,6 +208,7 @@ struct mkde_ctx {
     DevBuf<uint8_t> valid;
     DevBuf<int64_t> run_ptr;
     DevBuf<int2> run_se;
+    DevBuf<unsigned> run_cnt, run_first, run_chunks, run_total;   // building the runs
     DevBuf<float> colx, rowy;   // cell-centre tables of the mask grid
     DevBuf<uint8_t> bflags;
     DevBuf<float2> blist;
@@ -391,27 +402,14 @@ static int copy_d2h(mkde_ctx *c, void *dst, const void *src, size_t bytes)
 
 // ---------------------------------------------------------- point index --
 //
-// Uniform bins over the points' bounding box, about 64 points per bin, at most
-// 4M bins; a stable counting sort gives the bin order (ties keep the caller's
-// order, so every downstream summation order is deterministic).
+// Uniform bins over the points' bounding box, about pts_per_bin points per
+// bin, at most 4M bins.  The grid is sized here; each point's bin, the bin
+// order (a stable sort: ties keep the caller's order, so every downstream
+// summation order is deterministic), the bin starts and the bin-order
+// coordinates are built on the device (mkde_bins.hip).
 
-struct BinBuild {
-    BinGrid bg{};
-    int nbins = 0;
-    std::vector<int> perm;    // bin-order position -> caller's index
-    std::vector<int> start;   // nbins + 1
-};
-
-static void build_bins(const std::vector<float> &hx, const std::vector<float> &hy, int pts_per_bin, BinBuild &B)
+static BinGrid bin_grid(int64_t n, float x0, float x1, float y0, float y1, int pts_per_bin)
 {
-    const int64_t n = (int64_t)hx.size();
-    float x0 = FLT_MAX, x1 = -FLT_MAX, y0 = FLT_MAX, y1 = -FLT_MAX;
-    for (int64_t i = 0; i < n; ++i) {
-        x0 = std::min(x0, hx[i]);
-        x1 = std::max(x1, hx[i]);
-        y0 = std::min(y0, hy[i]);
-        y1 = std::max(y1, hy[i]);
-    }
     if (n == 0) { x0 = y0 = 0.0f; x1 = y1 = 1.0f; }
     double wx = std::max((double)x1 - x0, 1e-6), wy = std::max((double)y1 - y0, 1e-6);
     double target = std::max(1.0, (double)n / (double)pts_per_bin);
@@ -424,22 +422,7 @@ static void build_bins(const std::vector<float> &hx, const std::vector<float> &h
         if ((int64_t)nbx * nby <= (4 << 20)) break;
         size *= 1.25;
     }
-    B.bg = BinGrid{(double)x0, (double)y0, size, nbx, nby};
-    B.nbins = nbx * nby;
-    std::vector<int> key(n);
-    B.start.assign((size_t)B.nbins + 1, 0);
-    for (int64_t i = 0; i < n; ++i) {
-        int bx = (int)std::floor(((double)hx[i] - x0) / size);
-        int by = (int)std::floor(((double)hy[i] - y0) / size);
-        bx = std::min(std::max(bx, 0), nbx - 1);
-        by = std::min(std::max(by, 0), nby - 1);
-        key[i] = by * nbx + bx;
-        B.start[key[i] + 1]++;
-    }
-    for (int b = 0; b < B.nbins; ++b) B.start[b + 1] += B.start[b];
-    B.perm.assign(n, 0);
-    std::vector<int> pos(B.start.begin(), B.start.end() - 1);
-    for (int64_t i = 0; i < n; ++i) B.perm[pos[key[i]]++] = (int)i;
+    return BinGrid{(double)x0, (double)y0, size, nbx, nby};
 }
 
 // Drop every per-point state of the context (a failed upload leaves it
@@ -459,31 +442,33 @@ extern "C" int mkde_points_upload(mkde_ctx *c, int64_t n, const float *x, const 
     if (!c) return fail(MKDE_E_ARG, "null context");
     if (n < 0 || (n > 0 && (!x || !y))) return fail(MKDE_E_ARG, "mkde_points_upload: bad arguments");
     if (n > (int64_t)1 << 30) return fail(MKDE_E_RANGE, "too many points (%lld > 2^30)", (long long)n);
-    // validate before anything of the context changes: on this error the
-    // previous points stay usable
-    for (int64_t i = 0; i < n; ++i)
+    // validate (and take the bounding box) before anything of the context
+    // changes: on this error the previous points stay usable
+    float x0 = FLT_MAX, x1 = -FLT_MAX, y0 = FLT_MAX, y1 = -FLT_MAX;
+    for (int64_t i = 0; i < n; ++i) {
         if (!std::isfinite(x[i]) || !std::isfinite(y[i]))
             return fail(MKDE_E_ARG, "point %lld has a non-finite coordinate", (long long)i);
+        x0 = std::min(x0, x[i]);
+        x1 = std::max(x1, x[i]);
+        y0 = std::min(y0, y[i]);
+        y1 = std::max(y1, y[i]);
+    }
     CHK(set_device(c));
-    // build everything on the host in locals, then resize and fill the device
-    // buffers; the context takes the new points only when all of it succeeded
-    std::vector<float> hx, hy, hw, sx, sy;
-    BinBuild B;
+    const BinGrid bg = bin_grid(n, x0, x1, y0, y1, c->pts_per_bin);
+    const int nbins = bg.nbx * bg.nby;
+    // host copies (the rule of thumb runs on the host in point order); the
+    // context takes the new points only when all of the upload succeeded
+    std::vector<float> hx, hy, hw;
     try {
         hx.assign(x, x + n);
         hy.assign(y, y + n);
         if (w) hw.assign(w, w + n);
         else hw.assign(n, 1.0f);
-        build_bins(hx, hy, c->pts_per_bin, B);
-        sx.resize(n);
-        sy.resize(n);
-        for (int64_t i = 0; i < n; ++i) {
-            sx[i] = hx[B.perm[i]];
-            sy[i] = hy[B.perm[i]];
-        }
     } catch (const std::bad_alloc &) {
         return fail(MKDE_E_NOMEM, "out of host memory");
     }
+    size_t sort_bytes = 0;
+    HIPCHK(mkde_dev::bin_sort_bytes(n, nbins, &sort_bytes));
     forget_points(c);   // from here on a failure leaves the context without points
     HIPCHK(hipStreamSynchronize(c->stream));   // no queued work still reads the old buffers
     CHK(c->x.ensure(n));
@@ -492,10 +477,14 @@ extern "C" int mkde_points_upload(mkde_ctx *c, int64_t n, const float *x, const 
     CHK(c->xs.ensure(n));
     CHK(c->ys.ensure(n));
     CHK(c->perm.ensure(n));
-    CHK(c->bin_start.ensure(B.nbins + 1));
+    CHK(c->bin_key.ensure(n));
+    CHK(c->bin_skey.ensure(n));
+    CHK(c->bin_idx.ensure(n));
+    CHK(c->bin_tmp.ensure(std::max<size_t>(sort_bytes, 1)));
+    CHK(c->bin_start.ensure(nbins + 1));
     CHK(c->pts.ensure(n));
-    CHK(c->bin_box.ensure(B.nbins));
-    CHK(c->bin_thr.ensure(B.nbins));
+    CHK(c->bin_box.ensure(nbins));
+    CHK(c->bin_thr.ensure(nbins));
     CHK(c->reach.ensure(1));
     CHK(c->h.ensure(n));
     CHK(c->ew.ensure(n));
@@ -504,10 +493,8 @@ extern "C" int mkde_points_upload(mkde_ctx *c, int64_t n, const float *x, const 
     CHK(copy_h2d(c, c->x.p, hx.data(), n * sizeof(float)));
     CHK(copy_h2d(c, c->y.p, hy.data(), n * sizeof(float)));
     CHK(copy_h2d(c, c->w.p, hw.data(), n * sizeof(float)));
-    CHK(copy_h2d(c, c->xs.p, sx.data(), n * sizeof(float)));
-    CHK(copy_h2d(c, c->ys.p, sy.data(), n * sizeof(float)));
-    CHK(copy_h2d(c, c->perm.p, B.perm.data(), n * sizeof(int)));
-    CHK(copy_h2d(c, c->bin_start.p, B.start.data(), B.start.size() * sizeof(int)));
+    HIPCHK(mkde_dev::bin_points(c->stream, n, c->x.p, c->y.p, bg, nbins, c->bin_key.p, c->bin_skey.p, c->bin_idx.p,
+                                c->perm.p, c->bin_tmp.p, sort_bytes, c->xs.p, c->ys.p, c->bin_start.p));
     HIPCHK(mkde_dev::fill_f32(c->stream, n, 1.0f, c->ew.p));
     HIPCHK(mkde_dev::fill_f32(c->stream, n, 0.0f, c->h.p));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -515,11 +502,24 @@ extern "C" int mkde_points_upload(mkde_ctx *c, int64_t n, const float *x, const 
     c->hx.swap(hx);
     c->hy.swap(hy);
     c->hw.swap(hw);
-    c->hperm.swap(B.perm);
-    c->bg = B.bg;
-    c->nbins = B.nbins;
+    c->hperm.clear();   // read back from the device when needed (host_perm)
+    c->bg = bg;
+    c->nbins = nbins;
     c->n = n;
     return MKDE_OK;
+}
+
+// The bin order's permutation on the host (bin-order position -> caller's
+// index), read back once per upload.
+static int host_perm(mkde_ctx *c)
+{
+    if ((int64_t)c->hperm.size() == c->n) return MKDE_OK;
+    try {
+        c->hperm.resize(c->n);
+    } catch (const std::bad_alloc &) {
+        return fail(MKDE_E_NOMEM, "out of host memory");
+    }
+    return copy_d2h(c, c->hperm.data(), c->perm.p, c->n * sizeof(int));
 }
 
 extern "C" int mkde_num_points(mkde_ctx *c, int64_t *n)
@@ -646,14 +646,12 @@ extern "C" int mkde_mask_upload(mkde_ctx *c, const mkde_grid *g, const float *va
     if (!c) return fail(MKDE_E_ARG, "null context");
     CHK(check_grid(g));
     if (!values) return fail(MKDE_E_ARG, "mask values are NULL");
+    // run indices are 32-bit: a row has at most ceil(ncols / 2) runs
+    if ((uint64_t)g->nrows * (uint64_t)((g->ncols + 1) / 2) > UINT32_MAX)
+        return fail(MKDE_E_RANGE, "mask of %lld x %lld cells exceeds the run index (2^32 runs)",
+                    (long long)g->ncols, (long long)g->nrows);
     CHK(set_device(c));
     const int64_t nc = g->ncols * g->nrows;
-    RunsHost R;
-    try {
-        build_runs(values, g->ncols, g->nrows, g->nodata, R);
-    } catch (const std::bad_alloc &) {
-        return fail(MKDE_E_NOMEM, "out of host memory");
-    }
     // from here on a failure leaves the context without a mask (and without a
     // density tile that was this mask), never with buffers of another grid
     c->mask_set = false;
@@ -666,16 +664,24 @@ extern "C" int mkde_mask_upload(mkde_ctx *c, const mkde_grid *g, const float *va
     CHK(c->mask.ensure(nc));
     CHK(c->valid.ensure(nc));
     CHK(c->bflags.ensure(nc));
-    CHK(c->run_ptr.ensure(R.ptr.size()));
-    CHK(c->run_se.ensure(R.se.size()));
+    CHK(c->run_ptr.ensure(g->nrows + 1));
+    CHK(c->run_cnt.ensure(g->nrows));
+    CHK(c->run_first.ensure(g->nrows));
+    CHK(c->run_chunks.ensure((g->nrows + 1023) / 1024));
+    CHK(c->run_total.ensure(1));
     CHK(c->bcount.ensure(1));
     CHK(c->colx.ensure(g->ncols));
     CHK(c->rowy.ensure(g->nrows));
     CHK(copy_h2d(c, c->mask.p, values, nc * sizeof(float)));
-    CHK(copy_h2d(c, c->run_ptr.p, R.ptr.data(), R.ptr.size() * sizeof(int64_t)));
-    CHK(copy_h2d(c, c->run_se.p, R.se.data(), R.se.size() * sizeof(int2)));
     const GridDesc gd = to_desc(g);
+    // valid flags, then the valid-cell runs of every row, built on the device
     HIPCHK(mkde_dev::valid_cells(c->stream, nc, c->mask.p, g->nodata, c->valid.p));
+    HIPCHK(mkde_dev::runs_count(c->stream, gd, c->valid.p, c->run_cnt.p, c->run_first.p, c->run_chunks.p,
+                                c->run_total.p, c->run_ptr.p));
+    unsigned nruns = 0;
+    CHK(copy_d2h(c, &nruns, c->run_total.p, sizeof nruns));
+    CHK(c->run_se.ensure(std::max<int64_t>(nruns, 1)));
+    HIPCHK(mkde_dev::runs_fill(c->stream, gd, c->valid.p, c->run_first.p, c->run_se.p));
     HIPCHK(mkde_dev::centres(c->stream, gd, c->colx.p, c->rowy.p));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->mg = gd;
@@ -927,6 +933,7 @@ extern "C" int mkde_range_candidates(mkde_ctx *c, int64_t *offsets, int32_t *idx
     CHK(need_h(c));
     CHK(prep(c));
     const int64_t n = c->n;
+    CHK(host_perm(c));
     DevBuf<int64_t> counts;
     CHK(counts.ensure(n));
     PointIndex I = index_of(c);

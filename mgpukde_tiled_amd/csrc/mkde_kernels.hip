@@ -1613,6 +1613,64 @@ __global__ void k_valid(int64_t ncells, const float *mask, float nodata, uint8_t
     if (i < ncells) valid[i] = mask[i] != nodata ? 1 : 0;
 }
 
+// Valid-cell runs of the mask rows, one wave per row, columns 64 at a time:
+// a run starts at a valid cell whose left neighbour is not valid (or at
+// column 0) and ends at a valid cell whose right neighbour is not valid (or
+// at the last column).  k_run_count counts each row's runs; k_run_fill
+// writes them, in column order, from the row's first index (the exclusive
+// scan of the counts): a start is run number (starts before it in the row),
+// an end belongs to the last start at or before it.
+__device__ __forceinline__ void run_flags(const uint8_t *row, int64_t ncols, int64_t c, bool &start, bool &end)
+{
+    const bool v = c < ncols && row[c];
+    start = v && (c == 0 || !row[c - 1]);
+    end = v && (c == ncols - 1 || !row[c + 1]);
+}
+
+__global__ __launch_bounds__(256) void k_run_count(int64_t ncols, int64_t nrows, const uint8_t *valid, unsigned *cnt)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= nrows) return;   // whole wave
+    const uint8_t *row = valid + r * ncols;
+    unsigned n = 0;
+    for (int64_t c0 = 0; c0 < ncols; c0 += WAVE) {
+        bool s, e;
+        run_flags(row, ncols, c0 + lane, s, e);
+        n += (unsigned)__popcll(__ballot(s));
+    }
+    if (lane == 0) cnt[r] = n;
+}
+
+__global__ __launch_bounds__(256) void k_run_fill(int64_t ncols, int64_t nrows, const uint8_t *valid,
+                                                  const unsigned *first, int2 *se)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= nrows) return;   // whole wave
+    const uint8_t *row = valid + r * ncols;
+    const uint64_t le = ~0ull >> (63 - lane);   // lanes 0 .. lane
+    int64_t k = first[r];
+    for (int64_t c0 = 0; c0 < ncols; c0 += WAVE) {
+        const int64_t c = c0 + lane;
+        bool s, e;
+        run_flags(row, ncols, c, s, e);
+        const uint64_t ms = __ballot(s);
+        const int64_t upto = k + __popcll(ms & le);   // starts at columns <= c
+        if (s) se[upto - 1].x = (int)c;
+        if (e) se[upto - 1].y = (int)c;
+        k += __popcll(ms);
+    }
+}
+
+// run_ptr[r] = first index of row r's runs, run_ptr[nrows] = the total.
+__global__ void k_run_ptr(int64_t nrows, const unsigned *first, const unsigned *total, int64_t *ptr)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrows) ptr[r] = first[r];
+    else if (r == nrows) ptr[r] = *total;
+}
+
 // MarkBoundary: a valid cell is a boundary cell iff it is on the outer rows /
 // columns or one of its 8 neighbours is nodata (tiled:3993-4059).  Boundary
 // cell centres are appended to blist (order irrelevant: only minima are taken).
@@ -2619,6 +2677,25 @@ hipError_t range_fill(hipStream_t s, const PointIndex &idx, const int64_t *offse
 hipError_t valid_cells(hipStream_t s, int64_t ncells, const float *mask, float nodata, uint8_t *valid)
 {
     hipLaunchKernelGGL(k_valid, dim3(nblk(ncells, 256)), dim3(256), 0, s, ncells, mask, nodata, valid);
+    return hipGetLastError();
+}
+
+// Valid-cell runs of every row of a mask: per-row counts, their exclusive scan
+// (first[r]) and run_ptr; the total is left in *total for the caller to size
+// the run list before runs_fill.  first, cnt: nrows each; chunks:
+// (nrows + 1023) / 1024.
+hipError_t runs_count(hipStream_t s, const GridDesc &g, const uint8_t *valid, unsigned *cnt, unsigned *first,
+                      unsigned *chunks, unsigned *total, int64_t *ptr)
+{
+    hipLaunchKernelGGL(k_run_count, dim3(nblk(g.nrows, 4)), dim3(256), 0, s, g.ncols, g.nrows, valid, cnt);
+    scan_u32(s, (int)g.nrows, cnt, first, chunks, total);
+    hipLaunchKernelGGL(k_run_ptr, dim3(nblk_(g.nrows + 1)), dim3(256), 0, s, g.nrows, first, total, ptr);
+    return hipGetLastError();
+}
+
+hipError_t runs_fill(hipStream_t s, const GridDesc &g, const uint8_t *valid, const unsigned *first, int2 *se)
+{
+    hipLaunchKernelGGL(k_run_fill, dim3(nblk(g.nrows, 4)), dim3(256), 0, s, g.ncols, g.nrows, valid, first, se);
     return hipGetLastError();
 }
 

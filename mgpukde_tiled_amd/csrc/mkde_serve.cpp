@@ -118,14 +118,16 @@ extern "C" int64_t mkde_job_footprint(int64_t n, int64_t ncols, int64_t nrows, i
     const int64_t nbins = n / 32 + 8200;   // uniform bins: <= n/32 + 2*4096 + 1
     int64_t b = (int64_t)1 << 20;          // counters, per-candidate constants, reduction partials
     // coordinates (original and bin order), weights, permutation, point
-    // records, bandwidths, EC weights, distances, densities, work lists
-    b += 100 * n + 28 * nbins;
+    // records, bandwidths, EC weights, distances, densities, work lists, and
+    // the bin numbers and radix-sort storage that build the bin order
+    b += 128 * n + 28 * nbins;
     const bool loo = hoption == 1 || hoption == MKDE_H_SEARCH || (hoption == 2 && optimise);
     // saved bandwidths / EC weights and the per-candidate coefficient and
     // density arrays of the batched leave-one-out pass
     if (loo) b += 148 * n;
-    // mask, valid flags, boundary flags and list, row runs, density output
-    b += 24 * ncols * nrows + 16 * nrows;
+    // mask, valid flags, boundary flags and list, row runs, density output;
+    // per row and column: run pointers and counts, cell-centre tables
+    b += 24 * ncols * nrows + 32 * (nrows + ncols);
     // FP64 partials of split density blocks (8 B per cell): at most twice the
     // split target and 2 * cost / 2048 + 1 per block (cost <= n)
     const Tuning tune = mkde_dev::default_tuning();
