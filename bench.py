@@ -655,7 +655,7 @@ def run_sequential(jobs, plain=False):
     return out, t
 
 
-def bench_serving(n_jobs, slots, fractions=(0.5, 0.8, 0.95), plain=True, qps=0.0):
+def bench_serving(n_jobs, slots, fractions=(0.5, 0.8, 0.95), plain=True, qps=0.0, detail=None):
     """The mixed stream one job at a time (plain: one candidate per pass;
     batched: 16 candidates per pass) and served on `slots` slots: capacity
     from a burst (every job submitted at once), then Poisson arrivals at fixed
@@ -705,9 +705,23 @@ def bench_serving(n_jobs, slots, fractions=(0.5, 0.8, 0.95), plain=True, qps=0.0
                        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
                        "run_ms": {"p50": float(np.percentile(run, 50)), "p99": float(np.percentile(run, 99))},
                        "identical_results": same})
+        # latency by job size: the jobs with the shorter half of the run times
+        # alone (one at a time, batched) and the longer half
+        alone = np.array([a[0]["run_ms"] for a in seq])
+        small = alone <= np.median(alone)
+        for name, m in (("small", small), ("large", ~small)):
+            if m.any():
+                levels[-1]["latency_ms_" + name] = {"jobs": int(m.sum()),
+                                                    "p50": float(np.percentile(lat[m], 50)),
+                                                    "p90": float(np.percentile(lat[m], 90)),
+                                                    "p99": float(np.percentile(lat[m], 99))}
         log("serve: %.2f QPS: p50 %.0f ms, p99 %.0f ms" % (q, levels[-1]["latency_ms"]["p50"],
                                                           levels[-1]["latency_ms"]["p99"]))
     out["at_fixed_qps"] = levels
+    if detail:
+        with open(detail, "w") as f:
+            json.dump([{"n": j["n"], "side": j["side"], "hoption": j["hoption"], "run_ms_alone": a[0]["run_ms"]}
+                       for j, a in zip(jobs, seq)], f)
     return out
 
 
@@ -857,6 +871,7 @@ def main():
                     help="jobs of the serving sample in the benchmark line (0: none)")
     ap.add_argument("--slots", type=int, default=4, help="serve: worker slots")
     ap.add_argument("--qps", type=float, default=0.0, help="serve: one arrival rate instead of fractions of capacity")
+    ap.add_argument("--detail", default="", help="serve: write per-job sizes and run times to this file")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the one-candidate-per-pass baseline")
     ap.add_argument("--fractions", default="0.5,0.8,0.95",
                     help="serve: arrival rates as fractions of the measured capacity (empty: capacity only)")
@@ -881,7 +896,8 @@ def main():
         return
     if args.workload == "serve":
         fr = tuple(float(v) for v in args.fractions.split(",") if v.strip())
-        print(json.dumps(bench_serving(args.jobs, args.slots, fractions=fr, qps=args.qps, plain=not args.no_plain)),
+        print(json.dumps(bench_serving(args.jobs, args.slots, fractions=fr, qps=args.qps, plain=not args.no_plain,
+                                       detail=args.detail or None)),
               flush=True)
         return
     out, O_cpu = bench_adaptive(args)

@@ -6,7 +6,7 @@
 // Each slot is a host thread that owns one mkde_ctx, hence one HIP stream and
 // its own device buffers, grown to the largest job the slot has run and
 // reused (no hipMalloc/hipFree in steady state).  While one slot's job runs
-// kernels, another's does its host work (point binning, mask runs, copies),
+// kernels, another's does its host work (input checks, copies, waits),
 // and small jobs' kernels fill the device side by side.  Jobs start in
 // submission order when a slot is free and the HBM admission test passes:
 //
