@@ -1933,22 +1933,30 @@ __device__ __forceinline__ void narrow_box(const GridDesc &g, float px, float py
     cu = (int)(int32_t)(uint32_t)((nC - 1 < kr) ? nC - 1 : kr);
 }
 
-// Column centres of the mask grid: from a per-mask table (EC) or computed
-// (count pass on a tile); both give exactly cell_x.
-struct ColTable {
-    const float *x;
-    __device__ __forceinline__ float operator()(int64_t c) const { return x[c]; }
-};
+// Column centres of the mask grid: computed (cell_x) for ColCompute, and
+// dx = fl32(cell_x(c) - px) of a point: computed (DxCompute) or from the
+// point's table in LDS (DxLds, written while its prefix sums are formed);
+// both give the same dx.
 struct ColCompute {
     double xll, cs;
     // columns of one raster row fit in 32 bits (one-instruction conversion)
     __device__ __forceinline__ float operator()(int64_t c) const { return cell_x(xll, cs, (int64_t)(int)c); }
 };
+struct DxCompute {
+    double xll, cs;
+    float px;
+    __device__ __forceinline__ float operator()(int64_t c) const { return ColCompute{xll, cs}(c) - px; }
+};
+struct DxLds {
+    const float *dx;   // dx of columns c0, c0 + 1, ...
+    int c0;
+    __device__ __forceinline__ float operator()(int64_t c) const { return dx[(int)c - c0]; }
+};
 
-template <class COL>
-__device__ __forceinline__ bool pred_col(const COL &colx, int64_t c, float px, double dy2, double thr)
+template <class DX>
+__device__ __forceinline__ bool pred_col(const DX &dxf, int64_t c, float px, double dy2, double thr)
 {
-    const float dx = colx(c) - px;
+    const float dx = dxf(c);
     return __dadd_rn(sqd(dx), dy2) < thr;
 }
 
@@ -1961,8 +1969,8 @@ __device__ __forceinline__ bool pred_col(const COL &colx, int64_t c, float px, d
 // table loads of their centres are in flight at once; when the seed is off,
 // the row is empty unless cn is inside, and the ends are moved by the exact
 // predicate until they are the last columns inside.
-template <class COL>
-__device__ __forceinline__ void row_interval(const COL &colx, double xll, double inv_cs, int cl, int cu, int cn,
+template <class DX>
+__device__ __forceinline__ void row_interval(const DX &colx, double xll, double inv_cs, int cl, int cu, int cn,
                                              float px, double dy2, double thr, int &lo, int &hi)
 {
     const double rem = thr - dy2;
@@ -2063,7 +2071,9 @@ __device__ __forceinline__ int64_t first_run_ending_at_or_after(const int2 *runs
 // mask's centre tables) rather than loaded, and the row's run range [rs, re)
 // and, for a single-run row, its run r0 come loaded ahead by ec_rows, so the
 // row's arithmetic waits on no memory.
-__device__ __forceinline__ void ec_row(const EcArgs &A, const double *s_pd, int ca, int cb, int cl, int cu, int cn,
+template <class DX>
+__device__ __forceinline__ void ec_row(const EcArgs &A, const DX &dxf, const double *s_pd, int ca, int cb, int cl,
+                                       int cu, int cn,
                                        int r, int64_t rs, int64_t re, int2 r0, float px, float py, double thr,
                                        double a, double xll, double inv_cs, double yll, double cs, double nrm05,
                                        double &acc, unsigned long long &cnt)
@@ -2072,7 +2082,7 @@ __device__ __forceinline__ void ec_row(const EcArgs &A, const double *s_pd, int 
     const double dy2 = sqd(cy - py);
     if (!(dy2 < thr)) return;
     int lo, hi;
-    row_interval(ColCompute{xll, cs}, xll, inv_cs, cl, cu, cn, px, dy2, thr, lo, hi);
+    row_interval(dxf, xll, inv_cs, cl, cu, cn, px, dy2, thr, lo, hi);
     if (lo < ca) lo = ca;
     if (hi > cb) hi = cb;
     if (lo > hi) return;
@@ -2102,9 +2112,10 @@ __device__ __forceinline__ void ec_row(const EcArgs &A, const double *s_pd, int 
 // range of the next row and the single run of the current one are loaded at
 // the top of an iteration, so their latency hides behind the row's interval
 // search.
-__device__ __forceinline__ void ec_rows(const EcArgs &A, const double *s_pd, int ca, int cb, int cl, int cu, int cn,
-                                        int r, int ru, int step, float px, float py, double thr, double a,
-                                        double &acc, unsigned long long &cnt)
+template <class DX>
+__device__ __forceinline__ void ec_rows(const EcArgs &A, const DX &dxf, const double *s_pd, int ca, int cb, int cl,
+                                        int cu, int cn, int r, int ru, int step, float px, float py, double thr,
+                                        double a, double &acc, unsigned long long &cnt)
 {
     const GridDesc &g = A.g;
     const double xll = g.xll, yll = g.yll, cs = g.cs, inv_cs = 1.0 / cs, nrm05 = (double)g.nrows - 0.5;
@@ -2122,7 +2133,8 @@ __device__ __forceinline__ void ec_rows(const EcArgs &A, const double *s_pd, int
             rs_n = A.runs.run_ptr[rn];
             re_n = A.runs.run_ptr[rn + 1];
         }
-        ec_row(A, s_pd, ca, cb, cl, cu, cn, r, rs, re, r0, px, py, thr, a, xll, inv_cs, yll, cs, nrm05, acc, cnt);
+        ec_row(A, dxf, s_pd, ca, cb, cl, cu, cn, r, rs, re, r0, px, py, thr, a, xll, inv_cs, yll, cs, nrm05, acc,
+               cnt);
         rs = rs_n;
         re = re_n;
     }
@@ -2180,7 +2192,8 @@ __device__ __forceinline__ void ec_point(const EcArgs &A, double *s_pd, int p)
             }
             __syncthreads();
             // rows
-            ec_rows(A, s_pd, ca, cb, cl, cu, cn, rl + lane, ru, WAVE, px, py, thr, a, acc, cnt);
+            ec_rows(A, DxCompute{xll, cs, px}, s_pd, ca, cb, cl, cu, cn, rl + lane, ru, WAVE, px, py, thr, a, acc,
+                    cnt);
             __syncthreads();
         }
     }
@@ -2210,11 +2223,14 @@ __global__ __launch_bounds__(64) void k_ec(EcArgs A)
 
 // Wide points (NARROW box wider than the column chunk of k_ec): one
 // 256-thread workgroup per point and the whole box's prefix sums in dynamic
-// LDS (8 B per column), so every row is visited once instead of once per
-// chunk.  Same row arithmetic as k_ec; the prefix is scanned across the
-// workgroup (four waves).
-__device__ __forceinline__ void ec_point_wide(const EcArgs &A, double *s_pd, double *s_wt, double *s_acc,
-                                              unsigned long long *s_cnt, int p)
+// LDS, so every row is visited once instead of once per chunk.  Same row
+// arithmetic as k_ec; with DXT the rows' predicates read each column's
+// dx = fl32(cell_x - px) from LDS (written with the prefix, 4 B per column
+// next to the 8 B sum) instead of recomputing the cell centre in FP64 (the
+// same value).  The prefix is scanned across the workgroup (four waves).
+template <bool DXT>
+__device__ __forceinline__ void ec_point_wide(const EcArgs &A, double *s_pd, float *s_dx, double *s_wt,
+                                              double *s_acc, unsigned long long *s_cnt, int p)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const GridDesc g = A.g;
@@ -2237,10 +2253,11 @@ __device__ __forceinline__ void ec_point_wide(const EcArgs &A, double *s_pd, dou
     for (int t = 0; t < per; ++t) {
         const int k = my0 + t;
         if (k < w) {
-            const float dx = ColCompute{xll, cs}(cl + k) - px;
+            const float dx = DxCompute{xll, cs, px}(cl + k);
             const float ex = fast_exp2((float)(sqd(dx) * a));
             run += (double)ex;
             s_pd[k] = run;
+            if (DXT) s_dx[k] = dx;
         }
     }
     // exclusive scan of the thread totals: within each wave, then the waves
@@ -2261,7 +2278,8 @@ __device__ __forceinline__ void ec_point_wide(const EcArgs &A, double *s_pd, dou
     __syncthreads();
     double acc = 0.0;
     unsigned long long cnt = 0;
-    ec_rows(A, s_pd, cl, cu, cl, cu, cn, rl + tid, ru, 256, px, py, thr, a, acc, cnt);
+    if (DXT) ec_rows(A, DxLds{s_dx, cl}, s_pd, cl, cu, cl, cu, cn, rl + tid, ru, 256, px, py, thr, a, acc, cnt);
+    else ec_rows(A, DxCompute{xll, cs, px}, s_pd, cl, cu, cl, cu, cn, rl + tid, ru, 256, px, py, thr, a, acc, cnt);
     for (int d = 32; d >= 1; d >>= 1) {
         acc += __shfl_xor(acc, d, WAVE);
         cnt += __shfl_xor(cnt, d, WAVE);
@@ -2282,15 +2300,17 @@ __device__ __forceinline__ void ec_point_wide(const EcArgs &A, double *s_pd, dou
 // One workgroup per point of the wide part of the list (stored from the back
 // of the list), items blockIdx.x, blockIdx.x + gridDim.x, ... below the wide
 // count (the device count when given, else one item per workgroup).
-__global__ __launch_bounds__(256) void k_ec_wide(EcArgs A, int64_t n)
+template <bool DXT>
+__global__ __launch_bounds__(256) void k_ec_wide(EcArgs A, int64_t n, int wmax)
 {
-    extern __shared__ double s_pd[];
+    extern __shared__ double s_pd[];   // wmax prefix sums, then (DXT) wmax dx values
+    float *s_dx = (float *)(s_pd + wmax);
     __shared__ double s_wt[4];
     __shared__ double s_acc[4];
     __shared__ unsigned long long s_cnt[4];
     const int lim = A.count ? A.count[1] : (int)gridDim.x;
     for (int it = blockIdx.x; it < lim; it += gridDim.x) {
-        ec_point_wide(A, s_pd, s_wt, s_acc, s_cnt, A.list[n - 1 - it]);
+        ec_point_wide<DXT>(A, s_pd, s_dx, s_wt, s_acc, s_cnt, A.list[n - 1 - it]);
         __syncthreads();   // the next point rewrites the LDS this one read
     }
 }
@@ -2323,7 +2343,7 @@ __global__ __launch_bounds__(64) void k_count_pairs(CountArgs A)
                 const double dy2 = sqd(cy - P.y);
                 if (!(dy2 < P.thr)) continue;
                 int lo, hi;
-                row_interval(ColCompute{xll, cs}, xll, 1.0 / cs, cl, cu, cn, P.x, dy2, P.thr, lo, hi);
+                row_interval(DxCompute{xll, cs, P.x}, xll, 1.0 / cs, cl, cu, cn, P.x, dy2, P.thr, lo, hi);
                 if (lo > hi) continue;
                 const int64_t rs = A.runs.run_ptr[r], re = A.runs.run_ptr[r + 1];
                 for (int64_t k = first_run_ending_at_or_after(A.runs.run_se, rs, re, lo); k < re; ++k) {
@@ -2752,8 +2772,22 @@ hipError_t centres(hipStream_t s, const GridDesc &g, float *colx, float *rowy)
     return hipGetLastError();
 }
 
-// Widest box k_ec_wide takes: its prefix sums fill at most 64,000 B of LDS.
+// Widest box k_ec_wide takes, and its LDS: per column the FP64 prefix sum
+// (8 B), at most 64,000 B, and up to EC_DXT_MAX columns also the point's dx
+// (4 B more per column).  The dx table saves the rows' FP64 cell centres
+// (~15% of a launch up to ~2,900 columns); wider boxes keep five workgroups
+// per CU without it (measured slower with it from ~3,200 columns).
+#define EC_DXT_MAX 2900
 int ec_wide_max() { return 8000; }
+static size_t ec_wide_lds(int w)
+{
+    return (size_t)w * (w <= EC_DXT_MAX ? sizeof(double) + sizeof(float) : sizeof(double));
+}
+static void launch_ec_wide(hipStream_t s, unsigned grid, int w, const EcArgs &A, int64_t n)
+{
+    if (w <= EC_DXT_MAX) hipLaunchKernelGGL(k_ec_wide<true>, dim3(grid), dim3(256), ec_wide_lds(w), s, A, n, w);
+    else hipLaunchKernelGGL(k_ec_wide<false>, dim3(grid), dim3(256), ec_wide_lds(w), s, A, n, w);
+}
 
 // Most columns of the NARROW box of a point at the fixed bandwidth h on grid
 // g, for every point within 3h of a valid cell (narrow_box: the box spans
@@ -2814,7 +2848,7 @@ hipError_t ec(hipStream_t s, int64_t n, int n_narrow, int n_wide, int max_w, con
     }
     if (n_wide > 0) {
         if (max_w < 1 || max_w > ec_wide_max()) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_ec_wide, dim3((unsigned)n_wide), dim3(256), (size_t)max_w * sizeof(double), s, A, n);
+        launch_ec_wide(s, (unsigned)n_wide, max_w, A, n);
     }
     return hipGetLastError();
 }
@@ -2838,10 +2872,10 @@ hipError_t ec_dev(hipStream_t s, int64_t n, const GridDesc &g, const MaskRuns &r
     else if (cw == 1024) hipLaunchKernelGGL(k_ec<1024>, dim3(gn), dim3(64), 0, s, A);
     else hipLaunchKernelGGL(k_ec<2048>, dim3(gn), dim3(64), 0, s, A);
     if (w_wide > cw_narrow) {
-        const size_t lds = (size_t)w_wide * sizeof(double);
+        const size_t lds = ec_wide_lds(w_wide);
         const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160u << 10) / (lds + 256)));
         const unsigned gw = (unsigned)std::min<int64_t>(n, 256 * per_cu);
-        hipLaunchKernelGGL(k_ec_wide, dim3(gw), dim3(256), lds, s, A, n);
+        launch_ec_wide(s, gw, w_wide, A, n);
     }
     return hipGetLastError();
 }
