@@ -1184,7 +1184,21 @@ __global__ __launch_bounds__(256) void k_point_density_w(PtsArgs A)
             const int bst = I.bin_start[b], ben = I.bin_start[b + 1];
             if (bst == ben) continue;
             const double lim = FIXED ? A.thr_fixed : I.bin_thr[b];
-            if (!__builtin_amdgcn_readfirstlane((int)(box_gap2(I.bin_box[b], mnx, mxx, mny, mxy) < lim))) continue;
+            const BinInfo bb = I.bin_box[b];
+            if (!__builtin_amdgcn_readfirstlane((int)(box_gap2(bb, mnx, mxx, mny, mxy) < lim))) continue;
+            // FIXED: every pair of the bin and the wave is inside the screen
+            // when the largest fl32 offsets give d2f below it (fl32 of a
+            // difference, the product and the fma are monotone in |offset|,
+            // so no pair's d2f exceeds this one): such a bin's sources need no
+            // compare and no select, and add the same terms.  LOO keeps the
+            // tests for the bin holding the wave's own queries.
+            bool all_in = false;
+            if (FIXED) {
+                const float gx = fmaxf(bb.x1 - mnx, mxx - bb.x0), gy = fmaxf(bb.y1 - mny, mxy - bb.y0);
+                all_in = __builtin_fmaf(gy, gy, gx * gx) < f_thr_lo &&
+                         (!LOO || (int64_t)ben <= q0 || (int64_t)bst >= q0 + 64);
+                all_in = __builtin_amdgcn_readfirstlane((int)all_in);
+            }
             // this part's share of the candidate sequence
             const int o0 = ord, o1 = ord + (ben - bst);
             ord = o1;
@@ -1195,7 +1209,23 @@ __global__ __launch_bounds__(256) void k_point_density_w(PtsArgs A)
                 const int c1 = c0 + 32 < en ? c0 + 32 : en;
                 // four records are loaded before their arithmetic, so the
                 // scalar loads overlap
-                if (FIXED) {
+                if (FIXED && all_in) {
+                    int si = c0;
+                    for (; si + 4 <= c1; si += 4) {
+                        const PtRec p0 = I.pts[si], p1 = I.pts[si + 1], p2 = I.pts[si + 2], p3 = I.pts[si + 3];
+                        for (const PtRec &p : {p0, p1, p2, p3}) {
+                            const float dx = p.x - qx, dy = p.y - qy;
+                            const float d2f = __builtin_fmaf(dy, dy, dx * dx);
+                            acc = __builtin_fmaf(fast_exp2(d2f * f_a), p.coef, acc);
+                        }
+                    }
+                    for (; si < c1; ++si) {
+                        const PtRec p = I.pts[si];
+                        const float dx = p.x - qx, dy = p.y - qy;
+                        const float d2f = __builtin_fmaf(dy, dy, dx * dx);
+                        acc = __builtin_fmaf(fast_exp2(d2f * f_a), p.coef, acc);
+                    }
+                } else if (FIXED) {
                     bool band = false;
                     auto body = [&](const PtRec &p, int si) {
                         const float dx = p.x - qx, dy = p.y - qy;
