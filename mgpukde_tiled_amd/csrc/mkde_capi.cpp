@@ -69,9 +69,10 @@ int ec_box_cols(const GridDesc &, float);
 Tuning default_tuning();
 hipError_t centres(hipStream_t, const GridDesc &, float *, float *);
 hipError_t ec(hipStream_t, int64_t, int, int, int, const GridDesc &, const MaskRuns &, const float *, const float *,
-              const float *, const float *, const float *, float, const int *, float *, unsigned long long *, int);
+              const float *, const float *, const float *, float, const int *, float *, unsigned long long *, int,
+              int);
 hipError_t ec_dev(hipStream_t, int64_t, const GridDesc &, const MaskRuns &, const float *, const float *, const float *,
-                  const float *, const float *, const int *, const int *, float *, int, int, int);
+                  const float *, const float *, const int *, const int *, float *, int, int, int, int);
 hipError_t count_pairs(hipStream_t, const GridDesc &, const MaskRuns &, const PointIndex &, unsigned long long *);
 int64_t slot_words();
 hipError_t fold_slots(hipStream_t, unsigned long long *);
@@ -204,6 +205,7 @@ struct mkde_ctx {
     // edge-correction mask
     bool mask_set = false;
     GridDesc mg{};
+    int mg_cx32 = 0;   // the mask's column centres are exact in FP32 (grid_cx_fp32)
     DevBuf<float> mask;
     DevBuf<uint8_t> valid;
     DevBuf<int64_t> run_ptr;
@@ -641,6 +643,25 @@ static GridDesc to_desc(const mkde_grid *g)
     return GridDesc{g->ncols, g->nrows, g->xll, g->yll, g->cellsize, g->nodata};
 }
 
+// Whether every column centre of grid g, fl32(xll + (c + 0.5) cs) with the
+// product and the sum rounded in FP64 (cell_x), equals fl32(fma(c + 0.5, cs,
+// xll)) computed in FP32: then the edge-correction kernels use the shorter
+// FP32 form.  Checked column by column (c + 0.5 is exact in FP32 below 2^23).
+#pragma clang fp contract(off)
+static bool grid_cx_fp32(const mkde_grid *g)
+{
+    if (g->ncols > ((int64_t)1 << 23)) return false;
+    for (int64_t col = 0; col < g->ncols; ++col) {
+        const double prod = ((double)col + 0.5) * (double)g->cellsize;
+        const double sum = (double)g->xll + prod;
+        const float f64 = (float)sum;
+        const float f32 = std::fma((float)col + 0.5f, g->cellsize, g->xll);
+        if (__builtin_bit_cast(uint32_t, f64) != __builtin_bit_cast(uint32_t, f32)) return false;
+    }
+    return true;
+}
+#pragma clang fp contract(on)
+
 extern "C" int mkde_mask_upload(mkde_ctx *c, const mkde_grid *g, const float *values)
 {
     if (!c) return fail(MKDE_E_ARG, "null context");
@@ -685,6 +706,7 @@ extern "C" int mkde_mask_upload(mkde_ctx *c, const mkde_grid *g, const float *va
     HIPCHK(mkde_dev::centres(c->stream, gd, c->colx.p, c->rowy.p));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->mg = gd;
+    c->mg_cx32 = grid_cx_fp32(g) ? 1 : 0;
     c->mask_set = true;
     return MKDE_OK;
 }
@@ -885,7 +907,7 @@ static int run_ec(mkde_ctx *c, const float *h, float hf, float *ew_out, unsigned
     MaskRuns R{c->run_ptr.p, c->run_se.p};
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     HIPCHK(mkde_dev::ec(c->stream, c->n, cnt3[0], cnt3[1], cnt3[2], c->mg, R, c->colx.p, c->rowy.p, c->x.p, c->y.p,
-                        h, hf, c->list.p, ew_out, c->ucount.p, c->tune.ec_chunk));
+                        h, hf, c->list.p, ew_out, c->ucount.p, c->tune.ec_chunk, c->mg_cx32));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     // the pair count: folded, and read back unless the caller reads it later
     // (fold_count: from the last word of ucount, after its own wait)
@@ -1180,7 +1202,7 @@ static int search_enqueue(mkde_ctx *c, int K, int ec_on, const PointIndex &I, in
                                            c->ewk.p, c->list.p, c->icount.p, cw_narrow, wcols[g0 + j]));
                 HIPCHK(mkde_dev::ec_dev(c->stream, n, c->mg, R, c->colx.p, c->rowy.p, c->x.p, c->y.p, hd,
                                         c->icount.p, c->list.p, c->ewk.p, c->tune.ec_chunk, cw_narrow,
-                                        wcols[g0 + j]));
+                                        wcols[g0 + j], c->mg_cx32));
             }
             HIPCHK(mkde_dev::coef_k_dev(c->stream, n, c->perm.p, c->w.p, c->ewk.p, hd,
                                         c->coefk.p + (MKDE_LM_K - kg) + j, MKDE_LM_K));
@@ -1269,7 +1291,8 @@ static int loo_loglik_graph(mkde_ctx *c, int K, const std::vector<int> &ord, con
                               (uint64_t)(uintptr_t)c->rowy.p, (uint64_t)c->mg.ncols, (uint64_t)c->mg.nrows,
                               (uint64_t)__builtin_bit_cast(uint32_t, c->mg.xll),
                               (uint64_t)__builtin_bit_cast(uint32_t, c->mg.yll),
-                              (uint64_t)__builtin_bit_cast(uint32_t, c->mg.cs), (uint64_t)c->bg.nbx,
+                              (uint64_t)__builtin_bit_cast(uint32_t, c->mg.cs), (uint64_t)c->mg_cx32,
+                              (uint64_t)c->bg.nbx,
                               (uint64_t)c->bg.nby, __builtin_bit_cast(uint64_t, c->bg.x0),
                               __builtin_bit_cast(uint64_t, c->bg.y0), __builtin_bit_cast(uint64_t, c->bg.size),
                               (uint64_t)(uintptr_t)c->list.p, (uint64_t)(uintptr_t)c->icount.p,

@@ -1936,6 +1936,7 @@ struct EcArgs {
     const int *list;        // active points
     float *ew;              // original order
     unsigned long long *n_eval;
+    int cx32;               // cell centres of the mask's columns are exact in FP32 (host-checked)
 };
 
 __device__ __forceinline__ float ec_bandwidth(const EcArgs &A, int p)
@@ -1972,10 +1973,19 @@ struct ColCompute {
     // columns of one raster row fit in 32 bits (one-instruction conversion)
     __device__ __forceinline__ float operator()(int64_t c) const { return cell_x(xll, cs, (int64_t)(int)c); }
 };
+// CX32: the grid's centres fl32(xll + (c + 0.5) cs) (FP64 arithmetic) equal
+// fl32(fma(c + 0.5, cs, xll)) in FP32 for every column (checked on the host
+// column by column: then the FP32 form, three instructions instead of five,
+// gives the same centres).
+template <bool CX32>
 struct DxCompute {
     double xll, cs;
     float px;
-    __device__ __forceinline__ float operator()(int64_t c) const { return ColCompute{xll, cs}(c) - px; }
+    __device__ __forceinline__ float operator()(int64_t c) const
+    {
+        if constexpr (CX32) return __builtin_fmaf((float)(int)c + 0.5f, (float)cs, (float)xll) - px;
+        else return ColCompute{xll, cs}(c) - px;
+    }
 };
 struct DxLds {
     const float *dx;   // dx of columns c0, c0 + 1, ...
@@ -2172,8 +2182,8 @@ __device__ __forceinline__ void ec_rows(const EcArgs &A, const DX &dxf, const do
 
 // EC weight of point p by one wave: column chunks of EC_CW with the FP64
 // prefix sums of Ex in s_pd, rows on lanes.
-template <int EC_CW>
-__device__ __forceinline__ void ec_point(const EcArgs &A, double *s_pd, int p)
+template <int EC_CW, bool CX32>
+__device__ __forceinline__ void ec_point_t(const EcArgs &A, double *s_pd, int p)
 {
     const int lane = threadIdx.x;
     const GridDesc g = A.g;
@@ -2203,7 +2213,7 @@ __device__ __forceinline__ void ec_point(const EcArgs &A, double *s_pd, int p)
             for (int t = 0; t < per; ++t) {
                 const int k = my0 + t;
                 if (k < w) {
-                    const float dx = ColCompute{xll, cs}(ca + k) - px;
+                    const float dx = DxCompute<CX32>{xll, cs, px}(ca + k);
                     const float ex = fast_exp2((float)(sqd(dx) * a));
                     run += (double)ex;
                     s_pd[k] = run;
@@ -2222,8 +2232,8 @@ __device__ __forceinline__ void ec_point(const EcArgs &A, double *s_pd, int p)
             }
             __syncthreads();
             // rows
-            ec_rows(A, DxCompute{xll, cs, px}, s_pd, ca, cb, cl, cu, cn, rl + lane, ru, WAVE, px, py, thr, a, acc,
-                    cnt);
+            ec_rows(A, DxCompute<CX32>{xll, cs, px}, s_pd, ca, cb, cl, cu, cn, rl + lane, ru, WAVE, px, py, thr, a,
+                    acc, cnt);
             __syncthreads();
         }
     }
@@ -2244,6 +2254,13 @@ __device__ __forceinline__ void ec_point(const EcArgs &A, double *s_pd, int p)
 // items blockIdx.x, blockIdx.x + gridDim.x, ... below the narrow count (the
 // device count when given, else one item per workgroup).
 template <int EC_CW>
+__device__ __forceinline__ void ec_point(const EcArgs &A, double *s_pd, int p)
+{
+    if (A.cx32) ec_point_t<EC_CW, true>(A, s_pd, p);
+    else ec_point_t<EC_CW, false>(A, s_pd, p);
+}
+
+template <int EC_CW>
 __global__ __launch_bounds__(64) void k_ec(EcArgs A)
 {
     __shared__ double s_pd[EC_CW];   // prefix sums of Ex over the chunk's columns
@@ -2258,9 +2275,9 @@ __global__ __launch_bounds__(64) void k_ec(EcArgs A)
 // dx = fl32(cell_x - px) from LDS (written with the prefix, 4 B per column
 // next to the 8 B sum) instead of recomputing the cell centre in FP64 (the
 // same value).  The prefix is scanned across the workgroup (four waves).
-template <bool DXT>
-__device__ __forceinline__ void ec_point_wide(const EcArgs &A, double *s_pd, float *s_dx, double *s_wt,
-                                              double *s_acc, unsigned long long *s_cnt, int p)
+template <bool DXT, bool CX32>
+__device__ __forceinline__ void ec_point_wide_t(const EcArgs &A, double *s_pd, float *s_dx, double *s_wt,
+                                                double *s_acc, unsigned long long *s_cnt, int p)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const GridDesc g = A.g;
@@ -2283,7 +2300,7 @@ __device__ __forceinline__ void ec_point_wide(const EcArgs &A, double *s_pd, flo
     for (int t = 0; t < per; ++t) {
         const int k = my0 + t;
         if (k < w) {
-            const float dx = DxCompute{xll, cs, px}(cl + k);
+            const float dx = DxCompute<CX32>{xll, cs, px}(cl + k);
             const float ex = fast_exp2((float)(sqd(dx) * a));
             run += (double)ex;
             s_pd[k] = run;
@@ -2309,7 +2326,8 @@ __device__ __forceinline__ void ec_point_wide(const EcArgs &A, double *s_pd, flo
     double acc = 0.0;
     unsigned long long cnt = 0;
     if (DXT) ec_rows(A, DxLds{s_dx, cl}, s_pd, cl, cu, cl, cu, cn, rl + tid, ru, 256, px, py, thr, a, acc, cnt);
-    else ec_rows(A, DxCompute{xll, cs, px}, s_pd, cl, cu, cl, cu, cn, rl + tid, ru, 256, px, py, thr, a, acc, cnt);
+    else ec_rows(A, DxCompute<CX32>{xll, cs, px}, s_pd, cl, cu, cl, cu, cn, rl + tid, ru, 256, px, py, thr, a, acc,
+                 cnt);
     for (int d = 32; d >= 1; d >>= 1) {
         acc += __shfl_xor(acc, d, WAVE);
         cnt += __shfl_xor(cnt, d, WAVE);
@@ -2330,6 +2348,14 @@ __device__ __forceinline__ void ec_point_wide(const EcArgs &A, double *s_pd, flo
 // One workgroup per point of the wide part of the list (stored from the back
 // of the list), items blockIdx.x, blockIdx.x + gridDim.x, ... below the wide
 // count (the device count when given, else one item per workgroup).
+template <bool DXT>
+__device__ __forceinline__ void ec_point_wide(const EcArgs &A, double *s_pd, float *s_dx, double *s_wt,
+                                              double *s_acc, unsigned long long *s_cnt, int p)
+{
+    if (A.cx32) ec_point_wide_t<DXT, true>(A, s_pd, s_dx, s_wt, s_acc, s_cnt, p);
+    else ec_point_wide_t<DXT, false>(A, s_pd, s_dx, s_wt, s_acc, s_cnt, p);
+}
+
 template <bool DXT>
 __global__ __launch_bounds__(256) void k_ec_wide(EcArgs A, int64_t n, int wmax)
 {
@@ -2373,7 +2399,7 @@ __global__ __launch_bounds__(64) void k_count_pairs(CountArgs A)
                 const double dy2 = sqd(cy - P.y);
                 if (!(dy2 < P.thr)) continue;
                 int lo, hi;
-                row_interval(DxCompute{xll, cs, P.x}, xll, 1.0 / cs, cl, cu, cn, P.x, dy2, P.thr, lo, hi);
+                row_interval(DxCompute<false>{xll, cs, P.x}, xll, 1.0 / cs, cl, cu, cn, P.x, dy2, P.thr, lo, hi);
                 if (lo > hi) continue;
                 const int64_t rs = A.runs.run_ptr[r], re = A.runs.run_ptr[r + 1];
                 for (int64_t k = first_run_ending_at_or_after(A.runs.run_se, rs, re, lo); k < re; ++k) {
@@ -2868,9 +2894,9 @@ Tuning default_tuning()
 // max_w columns of LDS).
 hipError_t ec(hipStream_t s, int64_t n, int n_narrow, int n_wide, int max_w, const GridDesc &g, const MaskRuns &runs,
               const float *colx, const float *rowy, const float *x, const float *y, const float *h, float h_fixed,
-              const int *list, float *ew, unsigned long long *n_eval, int cw)
+              const int *list, float *ew, unsigned long long *n_eval, int cw, int cx32)
 {
-    EcArgs A{g, runs, colx, rowy, x, y, h, h_fixed, nullptr, nullptr, list, ew, n_eval};
+    EcArgs A{g, runs, colx, rowy, x, y, h, h_fixed, nullptr, nullptr, list, ew, n_eval, cx32};
     if (n_narrow > 0) {
         if (cw == 512) hipLaunchKernelGGL(k_ec<512>, dim3((unsigned)n_narrow), dim3(64), 0, s, A);
         else if (cw == 1024) hipLaunchKernelGGL(k_ec<1024>, dim3((unsigned)n_narrow), dim3(64), 0, s, A);
@@ -2892,11 +2918,11 @@ hipError_t ec(hipStream_t s, int64_t n, int n_narrow, int n_wide, int max_w, con
 // (w_wide <= cw_narrow).
 hipError_t ec_dev(hipStream_t s, int64_t n, const GridDesc &g, const MaskRuns &runs, const float *colx,
                   const float *rowy, const float *x, const float *y, const float *h_dev, const int *count,
-                  const int *list, float *ew, int cw, int cw_narrow, int w_wide)
+                  const int *list, float *ew, int cw, int cw_narrow, int w_wide, int cx32)
 {
     if (n <= 0) return hipSuccess;
     if (w_wide > ec_wide_max()) return hipErrorInvalidValue;
-    EcArgs A{g, runs, colx, rowy, x, y, nullptr, 0.0f, h_dev, count, list, ew, nullptr};
+    EcArgs A{g, runs, colx, rowy, x, y, nullptr, 0.0f, h_dev, count, list, ew, nullptr, cx32};
     const unsigned gn = (unsigned)std::min<int64_t>(n, 256 * 32);   // 8 waves per SIMD
     if (cw == 512) hipLaunchKernelGGL(k_ec<512>, dim3(gn), dim3(64), 0, s, A);
     else if (cw == 1024) hipLaunchKernelGGL(k_ec<1024>, dim3(gn), dim3(64), 0, s, A);

@@ -326,3 +326,33 @@ def test_ec_weights_wide_boxes(ctx, orc, hs_range):
     _, ew64, one = orc.ec_weights(x, y, od, hs, g, mask)
     assert ne == one
     assert rel_err(ew, ew64).max() <= REL
+
+
+@pytest.mark.parametrize("origin_cs", [(12.5, -7.25, 0.37), (1000.25, 50.0, 0.5), (300000.1, 4000000.0, 2.0),
+                                       (-55.0, 10.0, 1.0)])
+def test_ec_weights_grid_geometries(ctx, orc, origin_cs):
+    # the EC kernels form column centres in FP32 when the host finds every
+    # column's centre exact in FP32 (integer-ish grids), else in FP64: both
+    # must give the oracle's pair counts and weights (narrow and wide boxes)
+    import mgpukde_tiled_amd as M
+
+    xll, yll, cs = origin_cs
+    nc, nr = 700, 500
+    g = orc.grid(nc, nr, xll, yll, cs)
+    r = np.random.default_rng(23)
+    mask = np.ones((nr, nc), np.float32)
+    mask[r.random((nr, nc)) < 0.01] = -9999.0
+    mask[100:180, 200:420] = -9999.0
+    n = 200
+    x = (xll + r.uniform(0, nc * cs, n)).astype(np.float32)
+    y = (yll + r.uniform(0, nr * cs, n)).astype(np.float32)
+    hs = (cs * r.uniform(3.0, 260.0, n)).astype(np.float32)
+    ctx.points_upload(x, y, None)
+    ctx.mask_upload(mgrid(M, g), mask)
+    ctx.set_bandwidths(hs)
+    ew, ne = ctx.EdgeCorrectionWeightsExact()
+    oflags, _ = orc.mark_boundary(g, mask)
+    od = orc.dist_to_boundary(x, y, g, oflags)
+    _, ew64, one = orc.ec_weights(x, y, od, hs, g, mask)
+    assert ne == one
+    assert rel_err(ew, ew64).max() <= REL
