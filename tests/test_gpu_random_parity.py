@@ -98,3 +98,29 @@ def test_random_configuration(ctx, orc, k):
     pos = oden > 0
     assert np.all(np.asarray(den)[~pos] == 0)
     assert rel(np.asarray(den)[pos], oden[pos]).max(initial=0.0) <= REL
+
+
+@pytest.mark.parametrize("k", range(0, N_CASES, 2))
+def test_random_bandwidth_search(ctx, orc, k):
+    # the batched leave-one-out log-likelihood of five candidates around the
+    # case's bandwidth (EC recomputed per candidate, one pass, captured graph)
+    # against the FP64 oracle's EC weights, densities and log sum
+    import mgpukde_tiled_amd as M
+
+    c = make_case(k)
+    g = orc.grid(c["nc"], c["nr"], c["xll"], c["yll"], c["cs"])
+    x, y, w, mask = c["x"], c["y"], c["w"], c["mask"]
+    ctx.points_upload(x, y, w)
+    ctx.mask_upload(M.grid(c["nc"], c["nr"], c["xll"], c["yll"], c["cs"]), mask)
+    h = float(np.median(c["hs"]))
+    hc = (h * np.array([0.5, 0.8, 1.0, 1.3, 2.0])).astype(np.float32)
+    got = ctx.LogLikelihood(hc, edge_correction=True)
+    flags, _ = orc.mark_boundary(g, mask)
+    d = orc.dist_to_boundary(x, y, g, flags)
+    ref = np.empty(len(hc))
+    for j, hj in enumerate(hc):
+        hs = np.full(len(x), hj, np.float32)
+        _, ew64, _ = orc.ec_weights(x, y, d, hs, g, mask)
+        ref[j] = orc.loglik_fp64(orc.loo_density(x, y, w, ew64.astype(np.float32), hs)["den1_64"])
+    # each density within 2e-6 relative (FP32 sums folded every 32 terms)
+    assert np.max(np.abs(got - ref)) <= 2e-6 * len(x)
